@@ -1,0 +1,221 @@
+/*
+ * mpgos.h -- C ABI of the MI355X-native MPGOS solver library (libmpgos.so).
+ *
+ * The reference exposes its solver path as two C++ class templates,
+ *   ProblemSolver<NT,SD,NCP,NSP,NISP,NE,NA,NIA,NDO,Algorithm,Precision>
+ *     (SourceCodes/SingleSystem_PerThread_Interface.cuh:117-208)
+ *   ProblemSolver<NS,UPS,UD,TPB,SPB,NC,CBW,CCI,NUP,NSP,NGP,NiGP,NUA,NiUA,NSA,NiSA,NE,NDO,Algorithm,Precision>
+ *     (SourceCodes/CoupledSystems_PerBlock_Interface.cuh:135-248)
+ * with user callbacks bound at compile time.  Callbacks cannot cross a run-time
+ * C ABI, so every system is compiled into the library (see the registry in
+ * csrc/systems/) and selected by name at creation; everything else is a plain
+ * function over an opaque handle, plain pointers and sizes.  Each entry point
+ * names the member function of the reference it replaces.
+ *
+ * Error convention: the reference calls exit() on CUDA errors, bad indices and
+ * bad options (SingleSystem_PerThread_Interface.cuh:13-22, 565-581, 659-663).
+ * Here every call returns MPGOS_OK (0) or a negative MPGOS_E_* code and leaves
+ * a message for mpgos_last_error(); the C++ drop-in facade turns a failure back
+ * into the reference's exit(EXIT_FAILURE).
+ *
+ * Threading: one host thread per handle, as in the reference.  All transfers
+ * and launches are asynchronous on the handle's own HIP stream.
+ */
+#ifndef MPGOS_H
+#define MPGOS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ codes */
+enum {
+    MPGOS_OK = 0,
+    MPGOS_E_ARG = -1,      /* bad index, option or variable (reference: BoundCheck -> exit) */
+    MPGOS_E_HIP = -2,      /* HIP runtime error (reference: gpuErrCHK -> exit(1)) */
+    MPGOS_E_MEMORY = -3,   /* not enough device / pinned memory (reference: PT-IF:388-396) */
+    MPGOS_E_SYSTEM = -4,   /* unknown system name or unsupported algorithm/precision */
+    MPGOS_E_STATE = -5     /* call not valid in the current state (e.g. adjoint without forward) */
+};
+
+/* Algorithms, same order as enum Algorithms (SingleSystem_PerThread_Interface.cuh:33) */
+enum { MPGOS_RK4 = 0, MPGOS_RKCK45 = 1 };
+
+/* Arithmetic mode.  DOUBLE and FLOAT are the reference's Precision template
+ * argument; FLOAT_KAHAN is new: FP32 state and stages with compensated (Kahan)
+ * accumulation of time and state, FP64 error norm, step-size controller and
+ * event step control. */
+enum { MPGOS_DOUBLE = 0, MPGOS_FLOAT = 1, MPGOS_FLOAT_KAHAN = 2 };
+
+/* Per-thread ListOfVariables, same names and order as
+ * SingleSystem_PerThread_Interface.cuh:35-47; entries from PT_ACCEPTED_STEPS on
+ * are new per-instance outputs (counters, status, event log). */
+enum {
+    MPGOS_PT_ALL = 0,
+    MPGOS_PT_TIME_DOMAIN,
+    MPGOS_PT_ACTUAL_STATE,
+    MPGOS_PT_ACTUAL_TIME,
+    MPGOS_PT_CONTROL_PARAMETERS,
+    MPGOS_PT_SHARED_PARAMETERS,
+    MPGOS_PT_INTEGER_SHARED_PARAMETERS,
+    MPGOS_PT_ACCESSORIES,
+    MPGOS_PT_INTEGER_ACCESSORIES,
+    MPGOS_PT_DENSE_OUTPUT,
+    MPGOS_PT_DENSE_INDEX,
+    MPGOS_PT_DENSE_TIME,
+    MPGOS_PT_DENSE_STATE,
+    /* new */
+    MPGOS_PT_ACCEPTED_STEPS,   /* int64 [NT]: accepted steps of the last Solve */
+    MPGOS_PT_REJECTED_STEPS,   /* int64 [NT]: rejected attempts of the last Solve */
+    MPGOS_PT_STATUS,           /* int32 [NT]: MPGOS_STATUS_* bits of the last Solve */
+    MPGOS_PT_EVENT_COUNT,      /* int32 [NT]: detected events (all indices) in the last Solve */
+    MPGOS_PT_EVENT_LOG,        /* P [NT*(1+SD)*cap]: (t, x_0..x_SD-1) per logged hit, SoA */
+    MPGOS_PT_EVENT_LOG_INDEX,  /* int32 [NT*cap]: event function index of each logged hit */
+    MPGOS_PT_NUM_VARIABLES
+};
+
+/* Per-thread ListOfSolverOptions, same names and order as
+ * SingleSystem_PerThread_Interface.cuh:49-62; entries from PT_KERNEL_MODE on are new. */
+enum {
+    MPGOS_PT_THREADS_PER_BLOCK = 0,
+    MPGOS_PT_INITIAL_TIME_STEP,
+    MPGOS_PT_ACTIVE_NUMBER_OF_THREADS,
+    MPGOS_PT_MAXIMUM_TIME_STEP,
+    MPGOS_PT_MINIMUM_TIME_STEP,
+    MPGOS_PT_TIME_STEP_GROW_LIMIT,
+    MPGOS_PT_TIME_STEP_SHRINK_LIMIT,
+    MPGOS_PT_RELATIVE_TOLERANCE,
+    MPGOS_PT_ABSOLUTE_TOLERANCE,
+    MPGOS_PT_EVENT_TOLERANCE,
+    MPGOS_PT_EVENT_DIRECTION,
+    MPGOS_PT_DENSE_OUTPUT_MINIMUM_TIME_STEP,
+    MPGOS_PT_DENSE_OUTPUT_SAVE_FREQUENCY,
+    MPGOS_PT_PREFER_SHARED_MEMORY,
+    /* new */
+    MPGOS_PT_KERNEL_MODE,          /* 0 = one lane per instance (reference form), 1 = persistent work queue */
+    MPGOS_PT_PERSISTENT_WAVES_PER_SIMD, /* persistent grid size in waves per SIMD (1..8), default from occupancy */
+    MPGOS_PT_GRAPH_CAPTURE,        /* 1 = Solve() replays a captured hipGraph of its launches */
+    MPGOS_PT_EVENT_LOG_CAPACITY,   /* hits logged per instance (allocates EVENT_LOG), default 0 */
+    MPGOS_PT_LAUNCH_VARIANT,       /* register-budget variant of the kernel (0 = default, 1..3 = waves-per-EU 2,4,8) */
+    MPGOS_PT_NUM_OPTIONS
+};
+
+/* MPGOS_PT_KERNEL_MODE values */
+enum { MPGOS_KERNEL_PLAIN = 0, MPGOS_KERNEL_PERSISTENT = 1 };
+
+/* Per-instance status bits, replacing the reference's device printf
+ * (ErrorControllers.cuh:15,66,73; EventHandling.cuh:38). */
+enum {
+    MPGOS_STATUS_NONFINITE_RK4 = 1,      /* RK4 state not finite: instance terminated */
+    MPGOS_STATUS_NONFINITE_MIN_STEP = 2, /* RKCK45 not finite even at minimum step: terminated */
+    MPGOS_STATUS_MIN_STEP_REACHED = 4,   /* step forced at the minimum size, tolerance not guaranteed */
+    MPGOS_STATUS_EVENT_OMITTED = 8       /* event step below the minimum step: detection omitted */
+};
+
+typedef struct mpgos_solver mpgos_solver;
+
+typedef struct {
+    int nt;   /* NT: number of instances */
+    int sd;   /* SD: system dimension */
+    int ncp;  /* NCP: control parameters per instance */
+    int nsp;  /* NSP: shared parameters */
+    int nisp; /* NISP: integer shared parameters */
+    int ne;   /* NE: event functions */
+    int na;   /* NA: accessories per instance */
+    int nia;  /* NIA: integer accessories per instance */
+    int ndo;  /* NDO: dense-output points per instance */
+    int algorithm;
+    int precision;
+    int has_vjp; /* system provides the right-hand-side vector-Jacobian product (adjoint) */
+} mpgos_pt_dims;
+
+/* ---------------------------------------------------------- per-thread module */
+
+/* ProblemSolver::ProblemSolver(int AssociatedDevice)
+ *   (SingleSystem_PerThread_Interface.cuh:309-518): allocates pinned host mirrors
+ *   and device buffers, creates the stream and event, applies the default options
+ *   (dt0 1e-2, dtmax 1e6, dtmin 1e-12, grow 5, shrink 0.1, rtol = atol = 1e-8,
+ *   event tolerance 1e-6, direction 0, PreferSharedMemory on).
+ *   `system` names a registered system (mpgos_pt_list_systems). */
+int mpgos_pt_create(const char* system, int algorithm, int precision, int nt, int ndo,
+                    int device, mpgos_solver** out);
+
+/* ProblemSolver::~ProblemSolver() (SingleSystem_PerThread_Interface.cuh:521-562) */
+int mpgos_pt_destroy(mpgos_solver* h);
+
+/* The template arguments of the instantiated ProblemSolver (PT-IF:117). */
+int mpgos_pt_get_dims(const mpgos_solver* h, mpgos_pt_dims* out);
+
+/* ProblemSolver::SolverOption(ListOfSolverOptions, T) and
+ * ProblemSolver::SolverOption(ListOfSolverOptions, int, T)
+ *   (SingleSystem_PerThread_Interface.cuh:610-703).  `index` is ignored by the
+ *   one-argument options. */
+int mpgos_pt_solver_option(mpgos_solver* h, int option, int index, double value);
+
+/* ProblemSolver::SetHost overloads (SingleSystem_PerThread_Interface.cuh:705-830).
+ *   problem: ProblemNumber (ignored for global scope), serial: SerialNumber or
+ *   component, step: TimeStepNumber (DenseState only).  Integer variables take
+ *   the value cast to int, as the reference does. */
+int mpgos_pt_set_host(mpgos_solver* h, int variable, int problem, int serial, int step, double value);
+
+/* ProblemSolver::GetHost overloads (SingleSystem_PerThread_Interface.cuh:967-1080). */
+int mpgos_pt_get_host(const mpgos_solver* h, int variable, int problem, int serial, int step, double* value);
+
+/* Bulk access to the pinned host mirror of one variable, in the reference's SoA
+ * layout (index problem + serial*NT [+ step*NT*SD]); the vectorised form of
+ * SetHost/GetHost.  Returns NULL on error; *nbytes gets the size. */
+void* mpgos_pt_host_buffer(mpgos_solver* h, int variable, size_t* nbytes);
+
+/* Device address of one variable's buffer (same layout), for callers that keep
+ * inputs resident in HBM and fill or read them with their own kernels. */
+void* mpgos_pt_device_buffer(mpgos_solver* h, int variable, size_t* nbytes);
+
+/* ProblemSolver::SynchroniseFromHostToDevice / SynchroniseFromDeviceToHost
+ *   (SingleSystem_PerThread_Interface.cuh:832-965); asynchronous on the stream. */
+int mpgos_pt_sync_h2d(mpgos_solver* h, int variable);
+int mpgos_pt_sync_d2h(mpgos_solver* h, int variable);
+
+/* ProblemSolver::Solve() (SingleSystem_PerThread_Interface.cuh:1279-1285):
+ *   asynchronous launch of the integration kernel on the handle's stream. */
+int mpgos_pt_solve(mpgos_solver* h);
+
+/* ProblemSolver::SynchroniseDevice / InsertSynchronisationPoint / SynchroniseSolver
+ *   (SingleSystem_PerThread_Interface.cuh:1288-1312). */
+int mpgos_pt_synchronise_device(mpgos_solver* h);
+int mpgos_pt_insert_synchronisation_point(mpgos_solver* h);
+int mpgos_pt_synchronise_solver(mpgos_solver* h);
+
+/* ProblemSolver::Print(ListOfVariables) and Print(DenseOutput, ThreadID)
+ *   (SingleSystem_PerThread_Interface.cuh:1111-1276): same file names and text
+ *   format (width 18, precision 10, scientific), written into `directory`
+ *   (NULL = current directory).  thread_id is used for DenseOutput only. */
+int mpgos_pt_print(mpgos_solver* h, int variable, int thread_id, const char* directory);
+
+/* Time of the last Solve() kernel(s) in milliseconds, measured with HIP events
+ * on the handle's stream (valid after SynchroniseSolver). */
+int mpgos_pt_last_solve_ms(const mpgos_solver* h, double* ms);
+
+/* HIP stream of the handle (hipStream_t), for callers that order their own work. */
+void* mpgos_pt_stream(mpgos_solver* h);
+
+/* Names of the registered per-thread systems, comma separated. */
+const char* mpgos_pt_list_systems(void);
+
+/* Message of the last failed call on this thread. */
+const char* mpgos_last_error(void);
+
+/* ---------------------------------------------------------- math probes
+ * Evaluate the library's elementary functions (0 cos, 1 sin, 2 exp, 3 log,
+ * 4 pow(x, y)) on the host, or on the device over device buffers, so tests can
+ * show that both sides produce the same bits. */
+int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, long n);
+int mpgos_math_eval_device(int fn, const double* x_dev, const double* y_dev, double* out_dev, long n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPGOS_H */

@@ -1,0 +1,188 @@
+// extern "C" entry points of libmpgos.so (declared in include/mpgos.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "mpgos.h"
+#include "mpgos_math.h"
+#include "registry.h"
+
+namespace mpgos {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace pt {
+static const Registration kSystems[] = {
+    {"duffing", make_pt_duffing},
+    {"duffing_q", make_pt_duffing_q},
+    {"duffing_eps", make_pt_duffing_eps},
+    {"duffing_dw", make_pt_duffing_dw},
+    {"lorenz", make_pt_lorenz},
+    {"affine2", make_pt_affine2},
+};
+} // namespace pt
+
+// Elementary-function probe kernel: same source as the solver's functions.
+__global__ void math_eval_kernel(int fn, const double* x, const double* y, double* out, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    switch (fn) {
+    case 0: out[i] = math::cos(x[i]); break;
+    case 1: out[i] = math::sin(x[i]); break;
+    case 2: out[i] = math::exp(x[i]); break;
+    case 3: out[i] = math::log(x[i]); break;
+    default: out[i] = math::pow(x[i], y[i]); break;
+    }
+}
+
+} // namespace mpgos
+
+struct mpgos_solver {
+    mpgos::pt::SolverBase* pt = nullptr;
+};
+
+using mpgos::set_error;
+
+extern "C" {
+
+const char* mpgos_last_error(void) { return mpgos::g_last_error.c_str(); }
+
+const char* mpgos_pt_list_systems(void)
+{
+    static std::string names;
+    if (names.empty()) {
+        for (const auto& r : mpgos::pt::kSystems) {
+            if (!names.empty()) names += ",";
+            names += r.name;
+        }
+    }
+    return names.c_str();
+}
+
+int mpgos_pt_create(const char* system, int algorithm, int precision, int nt, int ndo, int device, mpgos_solver** out)
+{
+    if (!out || !system) { set_error("mpgos_pt_create: null argument"); return MPGOS_E_ARG; }
+    *out = nullptr;
+    for (const auto& r : mpgos::pt::kSystems) {
+        if (std::strcmp(r.name, system) != 0) continue;
+        mpgos::pt::SolverBase* s = r.make(precision);
+        if (!s) { set_error("mpgos_pt_create: unsupported precision"); return MPGOS_E_SYSTEM; }
+        int rc = s->init(algorithm, nt, ndo, device);
+        if (rc != MPGOS_OK) { delete s; return rc; }
+        mpgos_solver* h = new mpgos_solver;
+        h->pt = s;
+        *out = h;
+        return MPGOS_OK;
+    }
+    set_error(std::string("mpgos_pt_create: unknown system '") + system + "' (registered: " + mpgos_pt_list_systems() + ")");
+    return MPGOS_E_SYSTEM;
+}
+
+#define MPGOS_CHECK_HANDLE(h)                                     \
+    do {                                                          \
+        if (!(h) || !(h)->pt) { set_error("null solver handle"); return MPGOS_E_ARG; } \
+    } while (0)
+
+int mpgos_pt_destroy(mpgos_solver* h)
+{
+    MPGOS_CHECK_HANDLE(h);
+    delete h->pt;
+    delete h;
+    return MPGOS_OK;
+}
+
+int mpgos_pt_get_dims(const mpgos_solver* h, mpgos_pt_dims* out)
+{
+    MPGOS_CHECK_HANDLE(h);
+    if (!out) { set_error("null dims pointer"); return MPGOS_E_ARG; }
+    return h->pt->dims(out);
+}
+
+int mpgos_pt_solver_option(mpgos_solver* h, int option, int index, double value)
+{
+    MPGOS_CHECK_HANDLE(h);
+    return h->pt->option(option, index, value);
+}
+
+int mpgos_pt_set_host(mpgos_solver* h, int variable, int problem, int serial, int step, double value)
+{
+    MPGOS_CHECK_HANDLE(h);
+    return h->pt->set_host(variable, problem, serial, step, value);
+}
+
+int mpgos_pt_get_host(const mpgos_solver* h, int variable, int problem, int serial, int step, double* value)
+{
+    MPGOS_CHECK_HANDLE(h);
+    if (!value) { set_error("null value pointer"); return MPGOS_E_ARG; }
+    return h->pt->get_host(variable, problem, serial, step, value);
+}
+
+void* mpgos_pt_host_buffer(mpgos_solver* h, int variable, size_t* nbytes)
+{
+    if (!h || !h->pt) { set_error("null solver handle"); return nullptr; }
+    return h->pt->host_buffer(variable, nbytes);
+}
+
+void* mpgos_pt_device_buffer(mpgos_solver* h, int variable, size_t* nbytes)
+{
+    if (!h || !h->pt) { set_error("null solver handle"); return nullptr; }
+    return h->pt->device_buffer(variable, nbytes);
+}
+
+int mpgos_pt_sync_h2d(mpgos_solver* h, int variable) { MPGOS_CHECK_HANDLE(h); return h->pt->sync(variable, true); }
+int mpgos_pt_sync_d2h(mpgos_solver* h, int variable) { MPGOS_CHECK_HANDLE(h); return h->pt->sync(variable, false); }
+int mpgos_pt_solve(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->solve(); }
+int mpgos_pt_synchronise_device(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->synchronise_device(); }
+int mpgos_pt_insert_synchronisation_point(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->insert_sync_point(); }
+int mpgos_pt_synchronise_solver(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->synchronise_solver(); }
+int mpgos_pt_print(mpgos_solver* h, int variable, int thread_id, const char* directory)
+{
+    MPGOS_CHECK_HANDLE(h);
+    return h->pt->print(variable, thread_id, directory);
+}
+int mpgos_pt_last_solve_ms(const mpgos_solver* h, double* ms)
+{
+    MPGOS_CHECK_HANDLE(h);
+    if (!ms) { set_error("null pointer"); return MPGOS_E_ARG; }
+    return h->pt->last_solve_ms(ms);
+}
+void* mpgos_pt_stream(mpgos_solver* h)
+{
+    if (!h || !h->pt) { set_error("null solver handle"); return nullptr; }
+    return h->pt->stream();
+}
+
+int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, long n)
+{
+    if (fn < 0 || fn > 4 || !x || !out || (fn == 4 && !y)) { set_error("mpgos_math_eval_host: bad arguments"); return MPGOS_E_ARG; }
+    for (long i = 0; i < n; i++) {
+        switch (fn) {
+        case 0: out[i] = mpgos::math::cos(x[i]); break;
+        case 1: out[i] = mpgos::math::sin(x[i]); break;
+        case 2: out[i] = mpgos::math::exp(x[i]); break;
+        case 3: out[i] = mpgos::math::log(x[i]); break;
+        default: out[i] = mpgos::math::pow(x[i], y[i]); break;
+        }
+    }
+    return MPGOS_OK;
+}
+
+int mpgos_math_eval_device(int fn, const double* x_dev, const double* y_dev, double* out_dev, long n)
+{
+    if (fn < 0 || fn > 4 || !x_dev || !out_dev || (fn == 4 && !y_dev) || n < 0) {
+        set_error("mpgos_math_eval_device: bad arguments");
+        return MPGOS_E_ARG;
+    }
+    if (n == 0) return MPGOS_OK;
+    const int block = 256;
+    const long grid = (n + block - 1) / block;
+    hipLaunchKernelGGL(mpgos::math_eval_kernel, dim3((unsigned)grid), dim3(block), 0, 0, fn, x_dev, y_dev, out_dev, n);
+    MPGOS_HIP_TRY(hipGetLastError());
+    MPGOS_HIP_TRY(hipDeviceSynchronize());
+    return MPGOS_OK;
+}
+
+} // extern "C"

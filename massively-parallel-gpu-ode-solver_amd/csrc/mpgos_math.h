@@ -1,0 +1,236 @@
+// Elementary functions used by the solver and by the registered systems, compiled
+// from this one source for the gfx950 device pass and for the host (g++ / clang).
+//
+// Why the project owns these: the reference calls the CUDA device libm (`pow` in
+// SourceCodes/SingleSystem_PerThread_ExplicitRungeKutta_ErrorControllers.cuh:51-53,
+// `cos` in the user right-hand sides, e.g.
+// Tutorials_SingleSystem_PerThread/T1_Reference/Reference_SystemDefinition.cuh:12).
+// Bit-identical accepted-step and event counts between the GPU and the host oracle
+// require that both sides evaluate the same function bit for bit, so every routine
+// here uses only IEEE add/sub/mul/div, explicit fused multiply-add and round-to-
+// nearest-even, all of which are correctly rounded on gfx950 and on x86-64.
+// Compile every user of this header with -ffp-contract=off.
+//
+// Methods (all published, coefficients fitted by gen_math_coeffs.py beside this file):
+//  * cos/sin: Cody-Waite reduction by pi/2 with a two-part constant applied with
+//    fma (exact first part), then near-minimax polynomials on [-pi/4, pi/4]
+//    in z = r^2 (Chebyshev fits; error below 2e-17 of the polynomial part).
+//  * exp: reduction by ln2 (two-part, fma), polynomial e^r = 1 + r + r^2 E(r),
+//    scaling by 2^n through the exponent field.
+//  * log: x = 2^k m, m in [sqrt(2)/2, sqrt(2)), f = m-1, s = f/(2+f),
+//    log(1+f) = f - (f^2/2 - s (f^2/2 + R(s^2))), the 2 atanh(s) form of
+//    Tang / Sun fdlibm, with R fitted here.
+//  * pow(x, y) = exp(y log x) for x > 0 (a few ulp for |y log x| < ~50, which
+//    covers the controller's x^(1/5) and x^(1/4)).
+// Accuracy against the platform libm is asserted in tests/test_math.py.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MPGOS_HD __host__ __device__
+#else
+#define MPGOS_HD
+#endif
+
+#define MPGOS_INLINE inline __attribute__((always_inline))
+
+namespace mpgos {
+namespace math {
+
+MPGOS_HD MPGOS_INLINE uint64_t bits(double x) { return __builtin_bit_cast(uint64_t, x); }
+MPGOS_HD MPGOS_INLINE double from_bits(uint64_t u) { return __builtin_bit_cast(double, u); }
+MPGOS_HD MPGOS_INLINE uint32_t bits(float x) { return __builtin_bit_cast(uint32_t, x); }
+MPGOS_HD MPGOS_INLINE float from_bits32(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+MPGOS_HD MPGOS_INLINE double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+MPGOS_HD MPGOS_INLINE float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+MPGOS_HD MPGOS_INLINE double rint(double x) { return __builtin_rint(x); }
+MPGOS_HD MPGOS_INLINE float rint(float x) { return __builtin_rintf(x); }
+MPGOS_HD MPGOS_INLINE double fabs(double x) { return __builtin_fabs(x); }
+MPGOS_HD MPGOS_INLINE float fabs(float x) { return __builtin_fabsf(x); }
+MPGOS_HD MPGOS_INLINE bool isfinite(double x) { return (bits(x) & 0x7ff0000000000000ull) != 0x7ff0000000000000ull; }
+MPGOS_HD MPGOS_INLINE bool isfinite(float x) { return (bits(x) & 0x7f800000u) != 0x7f800000u; }
+MPGOS_HD MPGOS_INLINE bool isnan(double x) { return x != x; }
+
+// 2^n for integer n in [-1022, 1023] built from the exponent field.
+MPGOS_HD MPGOS_INLINE double pow2i(int n) { return from_bits((uint64_t)(n + 1023) << 52); }
+
+// ---------------------------------------------------------------- cos / sin
+namespace detail {
+// pi/2 = PIO2_HI + PIO2_LO (+ ~1e-33); PIO2_HI is the double nearest pi/2.
+constexpr double PIO2_HI = 0x1.921fb54442d18p+0;
+constexpr double PIO2_LO = 0x1.1a62633145c07p-54;
+constexpr double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+
+// cos(r) = 1 - z/2 + z^2 C(z), z = r^2, |r| <= pi/4 (+ rounding slack)
+MPGOS_HD MPGOS_INLINE double cos_kernel(double r)
+{
+    const double z = r * r;
+    double c = -0x1.907da367a37cbp-37;
+    c = fma(c, z, 0x1.1eeb68e93b64cp-29);
+    c = fma(c, z, -0x1.27e4fa17da09ep-22);
+    c = fma(c, z, 0x1.a01a019f4eb01p-16);
+    c = fma(c, z, -0x1.6c16c16c16967p-10);
+    c = fma(c, z, 0x1.5555555555555p-5);
+    // 1 - z/2 + z^2 c, with the large term kept separate until the end
+    const double hz = 0.5 * z;
+    const double w = 1.0 - hz;                    // exact for z <= 0.62 up to one rounding
+    return w + (((1.0 - w) - hz) + z * z * c);    // recover the rounding error of w
+}
+
+// sin(r) = r + r z S(z)
+MPGOS_HD MPGOS_INLINE double sin_kernel(double r)
+{
+    const double z = r * r;
+    double s = 0x1.5e0b19f8b1451p-33;
+    s = fma(s, z, -0x1.ae600b02b6262p-26);
+    s = fma(s, z, 0x1.71de37968a100p-19);
+    s = fma(s, z, -0x1.a01a019e83aaep-13);
+    s = fma(s, z, 0x1.1111111110bb2p-7);
+    s = fma(s, z, -0x1.5555555555555p-3);
+    return fma(r * z, s, r);
+}
+
+// r = x - n pi/2 and the quadrant n mod 4.
+MPGOS_HD MPGOS_INLINE double reduce_pio2(double x, int& quadrant)
+{
+    const double n = rint(x * TWO_OVER_PI);
+    double r = fma(-n, PIO2_HI, x);   // exact: granularity of n*PIO2_HI and x is >= 2^-52 and |r| < 1
+    r = fma(-n, PIO2_LO, r);
+    quadrant = (int)((int64_t)n & 3);
+    return r;
+}
+} // namespace detail
+
+MPGOS_HD MPGOS_INLINE double cos(double x)
+{
+    if (!isfinite(x)) return x - x;   // NaN for +-inf and NaN
+    int q;
+    const double r = detail::reduce_pio2(x, q);
+    double c = detail::cos_kernel(r);
+    double s = detail::sin_kernel(r);
+    double v = (q & 1) ? s : c;
+    return (q == 1 || q == 2) ? -v : v;
+}
+
+MPGOS_HD MPGOS_INLINE double sin(double x)
+{
+    if (!isfinite(x)) return x - x;
+    int q;
+    const double r = detail::reduce_pio2(x, q);
+    double c = detail::cos_kernel(r);
+    double s = detail::sin_kernel(r);
+    double v = (q & 1) ? c : s;
+    return (q >= 2) ? -v : v;
+}
+
+// ---------------------------------------------------------------- exp
+namespace detail {
+constexpr double LOG2E = 0x1.71547652b82fep+0;
+constexpr double LN2_HI = 0x1.62e42fefa39efp-1;    // double nearest ln 2
+constexpr double LN2_LO = 0x1.abc9e3b39803fp-56;   // ln 2 - LN2_HI
+
+// e^r = 1 + r + r^2 E(r), |r| <= ln2/2
+MPGOS_HD MPGOS_INLINE double exp_kernel(double r)
+{
+    double e = 0x1.1f72fc730b4ffp-29;
+    e = fma(e, r, 0x1.af4ddd84882fep-26);
+    e = fma(e, r, 0x1.27e4db67b4303p-22);
+    e = fma(e, r, 0x1.71de02375656cp-19);
+    e = fma(e, r, 0x1.a01a01a6d7808p-16);
+    e = fma(e, r, 0x1.a01a01abe62ddp-13);
+    e = fma(e, r, 0x1.6c16c16c162d6p-10);
+    e = fma(e, r, 0x1.11111111100dfp-7);
+    e = fma(e, r, 0x1.5555555555556p-5);
+    e = fma(e, r, 0x1.5555555555557p-3);
+    e = fma(e, r, 0.5);
+    return 1.0 + fma(r * r, e, r);
+}
+} // namespace detail
+
+MPGOS_HD MPGOS_INLINE double exp(double x)
+{
+    if (x != x) return x;
+    if (x > 709.782712893384) return from_bits(0x7ff0000000000000ull);
+    if (x < -745.1332191019412) return 0.0;
+    const double n = rint(x * detail::LOG2E);
+    double r = fma(-n, detail::LN2_HI, x);   // exact (see reduce_pio2)
+    r = fma(-n, detail::LN2_LO, r);
+    const double p = detail::exp_kernel(r);
+    const int k = (int)n;
+    // two-step scaling keeps 2^k representable near the overflow / subnormal ends
+    const int k1 = k / 2;
+    return (p * pow2i(k1)) * pow2i(k - k1);
+}
+
+// ---------------------------------------------------------------- log
+namespace detail {
+// 2 atanh(s) = 2 s + s R(z), z = s^2, R(z) = z * L(z), |s| <= (sqrt2-1)/(sqrt2+1)
+MPGOS_HD MPGOS_INLINE double log_R(double z)
+{
+    double l = 0x1.2b584aae78a57p-3;
+    l = fma(l, z, 0x1.39fe606542ddep-3);
+    l = fma(l, z, 0x1.7462b4ab2ef6bp-3);
+    l = fma(l, z, 0x1.c71c62e5800a1p-3);
+    l = fma(l, z, 0x1.2492492df148dp-2);
+    l = fma(l, z, 0x1.99999999952e2p-2);
+    l = fma(l, z, 0x1.5555555555558p-1);
+    return z * l;
+}
+} // namespace detail
+
+MPGOS_HD MPGOS_INLINE double log(double x)
+{
+    if (x != x) return x;
+    if (x < 0.0) return from_bits(0x7ff8000000000000ull);
+    if (x == 0.0) return -from_bits(0x7ff0000000000000ull);
+    if (!isfinite(x)) return x;
+    int k = 0;
+    if (x < 0x1p-1022) { x *= 0x1p54; k = -54; }       // subnormal input
+    uint64_t u = bits(x);
+    k += (int)((u >> 52) & 0x7ff) - 1023;
+    double m = from_bits((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);   // [1, 2)
+    if (m > 0x1.6a09e667f3bcdp+0) { m *= 0.5; k += 1; }                        // [sqrt2/2, sqrt2)
+    const double f = m - 1.0;                 // exact (Sterbenz)
+    const double s = f / (2.0 + f);
+    const double hfsq = 0.5 * f * f;
+    const double R = detail::log_R(s * s);
+    const double dk = (double)k;
+    const double lo = fma(dk, detail::LN2_LO, s * (hfsq + R));
+    return fma(dk, detail::LN2_HI, f - (hfsq - lo));
+}
+
+// x^y for the controller's use (x >= 0, y > 0 typical); IEEE special cases for
+// x in {0, inf, NaN}; negative x gives NaN unless y is an integer.
+MPGOS_HD MPGOS_INLINE double pow(double x, double y)
+{
+    if (x != x || y != y) return x + y;
+    if (y == 0.0) return 1.0;
+    if (x == 1.0) return 1.0;
+    if (x == 0.0) return y > 0.0 ? 0.0 : from_bits(0x7ff0000000000000ull);
+    if (!isfinite(x)) {
+        if (x > 0.0) return y > 0.0 ? x : 0.0;
+        return from_bits(0x7ff8000000000000ull);
+    }
+    if (x < 0.0) {
+        if (rint(y) != y) return from_bits(0x7ff8000000000000ull);
+        const double a = exp(y * log(-x));
+        const double half = 0.5 * y;
+        return (rint(half) != half) ? -a : a;   // odd integer exponent
+    }
+    return exp(y * log(x));
+}
+
+// ---------------------------------------------------------------- float versions
+// Used by the FP32 stepping mode: computed through the double routines and
+// rounded once, so float results are deterministic and within 1 ulp.
+MPGOS_HD MPGOS_INLINE float cos(float x) { return (float)cos((double)x); }
+MPGOS_HD MPGOS_INLINE float sin(float x) { return (float)sin((double)x); }
+MPGOS_HD MPGOS_INLINE float exp(float x) { return (float)exp((double)x); }
+MPGOS_HD MPGOS_INLINE float log(float x) { return (float)log((double)x); }
+MPGOS_HD MPGOS_INLINE float pow(float x, float y) { return (float)pow((double)x, (double)y); }
+
+} // namespace math
+} // namespace mpgos

@@ -1,0 +1,504 @@
+// Per-thread solver for gfx950: one lane integrates one instance.
+//
+// Semantics follow the reference kernel SingleSystem_PerThread
+// (SourceCodes/SingleSystem_PerThread_Solver.cuh:11-437) step for step:
+// prologue (loads, Initialization, first event evaluation, first dense store),
+// the loop  clip-to-end -> stepper -> controller -> event step control ->
+// [accept: t,x update, ActionAfterSuccessfulTimeStep, event detection and
+// ActionAfterEventDetection, event re-evaluation, dense output, termination],
+// then Finalization and write-back.  The numerics are
+//   RK4     SingleSystem_PerThread_ExplicitRungeKutta_Steppers.cuh:6-117
+//   RKCK45  SingleSystem_PerThread_ExplicitRungeKutta_Steppers.cuh:121-290
+//   controllers  SingleSystem_PerThread_ExplicitRungeKutta_ErrorControllers.cuh:5-86
+//   event step control  SingleSystem_PerThread_EventHandling.cuh:5-45
+//   dense output  SingleSystem_PerThread_DenseOutput.cuh:5-58
+// with every multiply-add of the stage combinations written as an explicit
+// fused multiply-add (the form a contracting compiler gives the reference's
+// expressions) and the rest compiled with -ffp-contract=off, so the FP64 result
+// is reproducible bit for bit by the host oracle.
+//
+// Design for MI355X (not present in the reference):
+//  * instance state lives in VGPRs; SoA HBM layout (index tid + c*NT) keeps
+//    every load/store of a wavefront contiguous;
+//  * tolerances and solver options arrive by value in the kernel argument
+//    block, so they sit in SGPRs; shared parameters are staged in LDS when
+//    PreferSharedMemory is on (the reference default);
+//  * device printf is replaced by per-instance status bits and counters;
+//  * a persistent form pulls instances from a device work queue and refills
+//    lanes whose instance finished, found with a wavefront ballot;
+//  * FLOAT_KAHAN mode: FP32 state and stages, compensated accumulation of
+//    time and state, FP64 error norm, controller and event step control.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <type_traits>
+
+#include "mpgos_math.h"
+#include "mpgos.h"
+
+namespace mpgos {
+namespace pt {
+
+constexpr int kMaxBlock = 256;
+
+template <int N> struct Pos { static constexpr int value = N > 0 ? N : 1; };
+
+// Kernel argument block.  CP is the precision of the step-size control.
+template <class Sys, class P, class CP>
+struct Args {
+    static constexpr int SD = Sys::SD;
+    static constexpr int NE = Sys::NE;
+    // SoA buffers (reference layout)
+    P* TD; P* X; P* T; P* cPAR; P* sPAR; int* sPARi; P* ACC; int* ACCi;
+    int* DOidx; P* DOt; P* DOx;
+    // per-instance outputs
+    long long* nacc; long long* nrej; int* status; int* nev; P* evlog; int* evlogidx;
+    unsigned int* queue;      // persistent form: [0] = next instance
+    int NT, nActive, NDO, evcap, preferShared;
+    int doFreq;
+    CP dt0, dtmax, dtmin, grow, shrink;
+    P doMinDt;
+    CP rtol[Pos<SD>::value], atol[Pos<SD>::value];
+    P etol[Pos<NE>::value];
+    int edir[Pos<NE>::value];
+};
+
+template <class Sys, class P, bool KAHAN, int ALG>
+struct Lane {
+    using CP = typename std::conditional<KAHAN, double, P>::type;
+    using A = Args<Sys, P, CP>;
+    static constexpr int SD = Sys::SD, NCP = Sys::NCP, NE = Sys::NE, NA = Sys::NA, NIA = Sys::NIA;
+
+    P TD[2];
+    P X[SD];
+    P Xc[KAHAN ? SD : 1];                 // Kahan compensation of the state
+    P cPAR[Pos<NCP>::value];
+    P ACC[Pos<NA>::value];
+    int ACCi[Pos<NIA>::value];
+    P EF[Pos<NE>::value];
+    P t;
+    P tc;                                 // Kahan compensation of the time
+    CP dt, newDt;
+    P doT;
+    int doIdx, skipped, term, udt;
+    int nacc, nrej, status, nev;
+    int tid;
+
+    // ------------------------------------------------------------ load / prologue
+    __device__ __forceinline__ void load(const A& a, int id, P* sPAR, int* sPARi)
+    {
+        tid = id;
+        const int NT = a.NT;
+#pragma unroll
+        for (int i = 0; i < 2; i++) TD[i] = a.TD[tid + i * NT];
+#pragma unroll
+        for (int i = 0; i < SD; i++) { X[i] = a.X[tid + i * NT]; if (KAHAN) Xc[i] = P(0); }
+#pragma unroll
+        for (int i = 0; i < NCP; i++) cPAR[i] = a.cPAR[tid + i * NT];
+#pragma unroll
+        for (int i = 0; i < NA; i++) ACC[i] = a.ACC[tid + i * NT];
+#pragma unroll
+        for (int i = 0; i < NIA; i++) ACCi[i] = a.ACCi[tid + i * NT];
+        t = a.T[tid];
+        tc = P(0);
+        dt = a.dt0;
+        newDt = a.dt0;
+        doIdx = a.DOidx[tid];
+        skipped = 0;
+        term = 0;
+        udt = 0;
+        nacc = 0; nrej = 0; status = 0; nev = 0;
+        doT = P(0);
+
+        // PT-SOL:148-193
+        P dtP = (P)dt;
+        Sys::template Initialization<P>(tid, NT, doIdx, t, dtP, TD, X, cPAR, sPAR, sPARi, ACC, ACCi);
+        dt = (CP)dtP;
+        if (NE > 0)
+            Sys::template EventFunction<P>(tid, NT, EF, t, dtP, TD, X, cPAR, sPAR, sPARi, ACC, ACCi);
+        if (a.NDO > 0 && doIdx < a.NDO) store_dense(a);
+    }
+
+    __device__ __forceinline__ void store_dense(const A& a)
+    {
+        // PT-DO:5-37 (store branch)
+        const int NT = a.NT;
+        a.DOt[tid + doIdx * NT] = t;
+        int k = tid + doIdx * NT * SD;
+#pragma unroll
+        for (int i = 0; i < SD; i++) { a.DOx[k] = X[i]; k += NT; }
+        doIdx++;
+        skipped = 0;
+        doT = mpgos_fmin(t + a.doMinDt, TD[1]);
+    }
+
+    static __device__ __forceinline__ double mpgos_fmin(double x, double y) { return __builtin_fmin(x, y); }
+    static __device__ __forceinline__ float mpgos_fmin(float x, float y) { return __builtin_fminf(x, y); }
+    static __device__ __forceinline__ double mpgos_fmax(double x, double y) { return __builtin_fmax(x, y); }
+    static __device__ __forceinline__ float mpgos_fmax(float x, float y) { return __builtin_fmaxf(x, y); }
+
+    // ------------------------------------------------------------ steppers
+    // RK4: PT-STP:6-117.  Returns next state in `nx`; in Kahan mode `inc` gets
+    // the increment dt/6 (k1 + 2k2 + 2k3 + k4).
+    __device__ __forceinline__ void rk4(const A& a, P h, P* nx, P* inc, int& isfin, P* sPAR, int* sPARi)
+    {
+        using math::fma;
+        P Xs[SD], k[SD];
+        const P dTp2 = P(0.5) * h;
+        const P dTp6 = P(1.0 / 6.0) * h;
+        Sys::template OdeFunction<P>(tid, a.NT, nx, X, t, cPAR, sPAR, sPARi, ACC, ACCi);
+        P T = t + dTp2;
+#pragma unroll
+        for (int i = 0; i < SD; i++) Xs[i] = fma(nx[i], dTp2, X[i]);
+        Sys::template OdeFunction<P>(tid, a.NT, k, Xs, T, cPAR, sPAR, sPARi, ACC, ACCi);
+#pragma unroll
+        for (int i = 0; i < SD; i++) { nx[i] = fma(P(2.0), k[i], nx[i]); Xs[i] = fma(k[i], dTp2, X[i]); }
+        Sys::template OdeFunction<P>(tid, a.NT, k, Xs, T, cPAR, sPAR, sPARi, ACC, ACCi);
+        T = t + h;
+#pragma unroll
+        for (int i = 0; i < SD; i++) { nx[i] = fma(P(2.0), k[i], nx[i]); Xs[i] = fma(k[i], h, X[i]); }
+        Sys::template OdeFunction<P>(tid, a.NT, k, Xs, T, cPAR, sPAR, sPARi, ACC, ACCi);
+#pragma unroll
+        for (int i = 0; i < SD; i++) {
+            const P s = nx[i] + k[i];
+            if (KAHAN) { inc[i] = dTp6 * s; nx[i] = X[i] + inc[i]; }
+            else nx[i] = fma(dTp6, s, X[i]);
+            if (!math::isfinite(nx[i])) isfin = 0;
+        }
+    }
+
+    // RKCK45: PT-STP:121-290.  err[] in control precision.
+    __device__ __forceinline__ void rkck45(const A& a, P h, P* nx, P* inc, CP* err, int& isfin, P* sPAR, int* sPARi)
+    {
+        using math::fma;
+        P Xs[SD], k1[SD], k2[SD], k3[SD], k4[SD], k5[SD], k6[SD];
+        Sys::template OdeFunction<P>(tid, a.NT, k1, X, t, cPAR, sPAR, sPARi, ACC, ACCi);
+
+        P T = fma(h, P(1.0 / 5.0), t);
+#pragma unroll
+        for (int i = 0; i < SD; i++) Xs[i] = fma(h, P(1.0 / 5.0) * k1[i], X[i]);
+        Sys::template OdeFunction<P>(tid, a.NT, k2, Xs, T, cPAR, sPAR, sPARi, ACC, ACCi);
+
+        T = fma(h, P(3.0 / 10.0), t);
+#pragma unroll
+        for (int i = 0; i < SD; i++) {
+            P s = P(3.0 / 40.0) * k1[i];
+            s = fma(P(9.0 / 40.0), k2[i], s);
+            Xs[i] = fma(h, s, X[i]);
+        }
+        Sys::template OdeFunction<P>(tid, a.NT, k3, Xs, T, cPAR, sPAR, sPARi, ACC, ACCi);
+
+        T = fma(h, P(3.0 / 5.0), t);
+#pragma unroll
+        for (int i = 0; i < SD; i++) {
+            P s = P(3.0 / 10.0) * k1[i];
+            s = fma(P(-9.0 / 10.0), k2[i], s);
+            s = fma(P(6.0 / 5.0), k3[i], s);
+            Xs[i] = fma(h, s, X[i]);
+        }
+        Sys::template OdeFunction<P>(tid, a.NT, k4, Xs, T, cPAR, sPAR, sPARi, ACC, ACCi);
+
+        T = t + h;
+#pragma unroll
+        for (int i = 0; i < SD; i++) {
+            P s = P(-11.0 / 54.0) * k1[i];
+            s = fma(P(5.0 / 2.0), k2[i], s);
+            s = fma(P(-70.0 / 27.0), k3[i], s);
+            s = fma(P(35.0 / 27.0), k4[i], s);
+            Xs[i] = fma(h, s, X[i]);
+        }
+        Sys::template OdeFunction<P>(tid, a.NT, k5, Xs, T, cPAR, sPAR, sPARi, ACC, ACCi);
+
+        T = fma(h, P(7.0 / 8.0), t);
+#pragma unroll
+        for (int i = 0; i < SD; i++) {
+            P s = P(1631.0 / 55296.0) * k1[i];
+            s = fma(P(175.0 / 512.0), k2[i], s);
+            s = fma(P(575.0 / 13824.0), k3[i], s);
+            s = fma(P(44275.0 / 110592.0), k4[i], s);
+            s = fma(P(253.0 / 4096.0), k5[i], s);
+            Xs[i] = fma(h, s, X[i]);
+        }
+        Sys::template OdeFunction<P>(tid, a.NT, k6, Xs, T, cPAR, sPAR, sPARi, ACC, ACCi);
+
+#pragma unroll
+        for (int i = 0; i < SD; i++) {
+            P s = P(37.0 / 378.0) * k1[i];
+            s = fma(P(250.0 / 621.0), k3[i], s);
+            s = fma(P(125.0 / 594.0), k4[i], s);
+            s = fma(P(512.0 / 1771.0), k6[i], s);
+            if (KAHAN) { inc[i] = h * s; nx[i] = X[i] + inc[i]; }
+            else nx[i] = fma(h, s, X[i]);
+
+            if (KAHAN) {
+                // FP64 error norm over the FP32 stages
+                double e = (37.0 / 378.0 - 2825.0 / 27648.0) * (double)k1[i];
+                e = math::fma(250.0 / 621.0 - 18575.0 / 48384.0, (double)k3[i], e);
+                e = math::fma(125.0 / 594.0 - 13525.0 / 55296.0, (double)k4[i], e);
+                e = math::fma(0.0 - 277.0 / 14336.0, (double)k5[i], e);
+                e = math::fma(512.0 / 1771.0 - 1.0 / 4.0, (double)k6[i], e);
+                err[i] = math::fma((double)h, math::fabs(e), 1e-18);
+            } else {
+                P e = P(37.0 / 378.0 - 2825.0 / 27648.0) * k1[i];
+                e = fma(P(250.0 / 621.0 - 18575.0 / 48384.0), k3[i], e);
+                e = fma(P(125.0 / 594.0 - 13525.0 / 55296.0), k4[i], e);
+                e = fma(P(0.0 - 277.0 / 14336.0), k5[i], e);
+                e = fma(P(512.0 / 1771.0 - 1.0 / 4.0), k6[i], e);
+                if (std::is_same<P, double>::value) err[i] = (CP)math::fma((double)h, (double)math::fabs(e), 1e-18);
+                else err[i] = (CP)((double)(h * math::fabs(e)) + 1e-18);   // float*float, then + double literal
+            }
+            if (!math::isfinite(nx[i]) || !math::isfinite(err[i])) isfin = 0;
+        }
+    }
+
+    // ------------------------------------------------------------ controllers
+    // PT-CTL:23-86
+    __device__ __forceinline__ void control_rkck45(const A& a, const P* nx, const CP* err, int& update, int& isfin)
+    {
+        CP R = CP(1e30);
+#pragma unroll
+        for (int i = 0; i < SD; i++) {
+            const CP tol = mpgos_fmax(a.rtol[i] * mpgos_fmax(math::fabs((CP)nx[i]), math::fabs((CP)X[i])), a.atol[i]);
+            update = update && (err[i] < tol);
+            R = mpgos_fmin(R, tol / err[i]);
+        }
+        CP fac;
+        if (update == 1) fac = CP(0.9) * math::pow(R, CP(1.0 / 5.0));
+        else fac = CP(0.9) * math::pow(R, CP(1.0 / 4.0));
+        if (!math::isfinite(fac)) isfin = 0;
+
+        if (isfin == 0) {
+            fac = a.shrink;
+            update = 0;
+            if (dt < a.dtmin * CP(1.01)) { term = 1; status |= MPGOS_STATUS_NONFINITE_MIN_STEP; }
+        } else {
+            if (dt < a.dtmin * CP(1.01)) { update = 1; status |= MPGOS_STATUS_MIN_STEP_REACHED; }
+        }
+        fac = mpgos_fmin(fac, a.grow);
+        fac = mpgos_fmax(fac, a.shrink);
+        newDt = dt * fac;
+        newDt = mpgos_fmin(newDt, a.dtmax);
+        newDt = mpgos_fmax(newDt, a.dtmin);
+    }
+
+    // PT-EVT:5-45
+    __device__ __forceinline__ void event_control(const A& a, const P* EFn, int& update)
+    {
+        CP evDt = dt;
+        int corrected = 0;
+        if (update == 1 && term == 0) {
+#pragma unroll
+            for (int i = 0; i < NE; i++) {
+                const P tol = a.etol[i];
+                if ((EF[i] > tol && EFn[i] < -tol && a.edir[i] <= 0) ||
+                    (EF[i] < -tol && EFn[i] > tol && a.edir[i] >= 0)) {
+                    evDt = mpgos_fmin(evDt, (-(CP)EF[i] / ((CP)EFn[i] - (CP)EF[i])) * dt);
+                    corrected = 1;
+                }
+            }
+        }
+        if (corrected == 1) {
+            if (evDt < a.dtmin) status |= MPGOS_STATUS_EVENT_OMITTED;
+            else { newDt = evDt; update = 0; }
+        }
+    }
+
+    // ------------------------------------------------------------ one attempted step
+    // One iteration of the loop at PT-SOL:197-394.
+    __device__ __forceinline__ void step(const A& a, P* sPAR, int* sPARi)
+    {
+        const int NT = a.NT;
+        int update = 1, isfin = 1, endReached = 0;
+        dt = newDt;
+        CP remaining;
+        if (KAHAN) remaining = (CP)TD[1] - ((CP)t - (CP)tc);
+        else remaining = (CP)(TD[1] - t);
+        if (dt > remaining) { dt = remaining; endReached = 1; }
+
+        const P h = (P)dt;
+        P nx[SD], inc[KAHAN ? SD : 1];
+        if (ALG == MPGOS_RK4) {
+            rk4(a, h, nx, inc, isfin, sPAR, sPARi);
+            // PT-CTL:5-20
+            if (isfin == 0) { term = 1; status |= MPGOS_STATUS_NONFINITE_RK4; }
+            newDt = a.dt0;
+        } else {
+            CP err[SD];
+            rkck45(a, h, nx, inc, err, isfin, sPAR, sPARi);
+            control_rkck45(a, nx, err, update, isfin);
+        }
+
+        P EFn[Pos<NE>::value];
+        P dtP = h;
+        if (NE > 0) {
+            Sys::template EventFunction<P>(tid, NT, EFn, t + h, dtP, TD, nx, cPAR, sPAR, sPARi, ACC, ACCi);
+            event_control(a, EFn, update);
+        }
+
+        if (update == 1) {
+            if (KAHAN) {
+                // compensated t += h and x += inc
+                P y = h - tc; P s = t + y; tc = (s - t) - y; t = s;
+#pragma unroll
+                for (int i = 0; i < SD; i++) { P yi = inc[i] - Xc[i]; P si = X[i] + yi; Xc[i] = (si - X[i]) - yi; X[i] = si; }
+            } else {
+                t += h;
+#pragma unroll
+                for (int i = 0; i < SD; i++) X[i] = nx[i];
+            }
+            Sys::template ActionAfterSuccessfulTimeStep<P>(tid, NT, udt, t, dtP, TD, X, cPAR, sPAR, sPARi, ACC, ACCi);
+            if (NE > 0) {
+#pragma unroll
+                for (int i = 0; i < NE; i++) {
+                    const P tol = a.etol[i];
+                    if ((EF[i] > tol && math::fabs(EFn[i]) < tol && a.edir[i] <= 0) ||
+                        (EF[i] < -tol && math::fabs(EFn[i]) < tol && a.edir[i] >= 0)) {
+                        if (nev < a.evcap) {
+                            const long long base = (long long)nev * (SD + 1);
+                            a.evlog[tid + base * NT] = t;
+#pragma unroll
+                            for (int c = 0; c < SD; c++) a.evlog[tid + (base + 1 + c) * NT] = X[c];
+                            a.evlogidx[tid + (long long)nev * NT] = i;
+                        }
+                        nev++;
+                        Sys::template ActionAfterEventDetection<P>(tid, NT, i, udt, t, dtP, TD, X, cPAR, sPAR, sPARi, ACC, ACCi);
+                    }
+                }
+                Sys::template EventFunction<P>(tid, NT, EFn, t, dtP, TD, X, cPAR, sPAR, sPARi, ACC, ACCi);
+#pragma unroll
+                for (int i = 0; i < NE; i++) EF[i] = EFn[i];
+            }
+            if (a.NDO > 0) {
+                // PT-DO:40-58 then PT-DO:5-37
+                int upd;
+                if (doIdx < a.NDO && doT < t && skipped >= a.doFreq - 1) upd = 1;
+                else upd = 0;
+                if (doIdx < a.NDO && (endReached == 1 || udt == 1)) upd = 1;
+                if (upd) store_dense(a);
+                else skipped++;
+            }
+            if (endReached == 1 || udt == 1) term = 1;
+            dt = (CP)dtP;   // a callback may have changed dT; it reaches Finalization (PT-SOL:398-410)
+            nacc++;
+        } else {
+            nrej++;
+        }
+    }
+
+    // ------------------------------------------------------------ epilogue
+    __device__ __forceinline__ void finish(const A& a, P* sPAR, int* sPARi)
+    {
+        const int NT = a.NT;
+        P dtP = (P)dt;
+        Sys::template Finalization<P>(tid, NT, doIdx, t, dtP, TD, X, cPAR, sPAR, sPARi, ACC, ACCi);
+#pragma unroll
+        for (int i = 0; i < 2; i++) a.TD[tid + i * NT] = TD[i];
+#pragma unroll
+        for (int i = 0; i < SD; i++) a.X[tid + i * NT] = X[i];
+#pragma unroll
+        for (int i = 0; i < NCP; i++) a.cPAR[tid + i * NT] = cPAR[i];
+#pragma unroll
+        for (int i = 0; i < NA; i++) a.ACC[tid + i * NT] = ACC[i];
+#pragma unroll
+        for (int i = 0; i < NIA; i++) a.ACCi[tid + i * NT] = ACCi[i];
+        a.T[tid] = t;
+        a.DOidx[tid] = doIdx;
+        a.nacc[tid] = nacc;
+        a.nrej[tid] = nrej;
+        a.status[tid] = status;
+        a.nev[tid] = nev;
+    }
+};
+
+// Stage shared parameters into LDS (PT-SOL:67-88) or point at global memory.
+template <class Sys, class P, class CP>
+__device__ __forceinline__ void stage_shared(const Args<Sys, P, CP>& a, P*& sPAR, int*& sPARi)
+{
+    constexpr int NSP = Sys::NSP, NISP = Sys::NISP;
+    extern __shared__ __align__(16) unsigned char mpgos_dyn_lds[];
+    if (a.preferShared && (NSP + NISP) > 0) {
+        P* lp = reinterpret_cast<P*>(mpgos_dyn_lds);
+        int* li = reinterpret_cast<int*>(lp + NSP);
+        for (int i = threadIdx.x; i < NSP; i += blockDim.x) lp[i] = a.sPAR[i];
+        for (int i = threadIdx.x; i < NISP; i += blockDim.x) li[i] = a.sPARi[i];
+        __syncthreads();
+        sPAR = lp;
+        sPARi = li;
+    } else {
+        sPAR = a.sPAR;
+        sPARi = a.sPARi;
+    }
+}
+
+// Reference form: lane = instance, whole integration in one launch.
+template <class Sys, class P, bool KAHAN, int ALG>
+__device__ __forceinline__ void plain_body(const Args<Sys, P, typename Lane<Sys, P, KAHAN, ALG>::CP>& a)
+{
+    P* sPAR; int* sPARi;
+    stage_shared(a, sPAR, sPARi);
+    const int tid = threadIdx.x + blockIdx.x * blockDim.x;
+    if (tid < a.nActive) {
+        Lane<Sys, P, KAHAN, ALG> L;
+        L.load(a, tid, sPAR, sPARi);
+        while (L.term == 0) L.step(a, sPAR, sPARi);
+        L.finish(a, sPAR, sPARi);
+    }
+}
+
+// Persistent form: each lane repeatedly takes the next instance from the
+// device queue.  A wavefront ballot finds lanes whose instance terminated;
+// one atomic per wavefront reserves as many new instances as there are idle
+// lanes and the lanes take consecutive indices by their rank in the ballot.
+// Per-instance arithmetic is independent of the lane it runs on, so results
+// are bit-identical to the plain form.  Every lane exits once the queue is
+// drained and its last instance has finished.
+template <class Sys, class P, bool KAHAN, int ALG>
+__device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane<Sys, P, KAHAN, ALG>::CP>& a)
+{
+    P* sPAR; int* sPARi;
+    stage_shared(a, sPAR, sPARi);
+    Lane<Sys, P, KAHAN, ALG> L;
+    bool busy = false;
+    bool drained = false;          // wave-uniform
+    const unsigned lane = __lane_id();
+    for (;;) {
+        if (!drained) {
+            const unsigned long long idle = __ballot(!busy);
+            if (idle != 0ull) {
+                const unsigned nidle = (unsigned)__popcll(idle);
+                unsigned base = 0;
+                if (lane == (unsigned)(__ffsll((long long)idle) - 1))
+                    base = atomicAdd(a.queue, nidle);
+                base = __shfl(base, __ffsll((long long)idle) - 1);
+                if (base + nidle >= (unsigned)a.nActive) drained = true;
+                if (!busy) {
+                    const unsigned rank = (unsigned)__popcll(idle & ((1ull << lane) - 1ull));
+                    const unsigned id = base + rank;
+                    if (id < (unsigned)a.nActive) {
+                        L.load(a, (int)id, sPAR, sPARi);
+                        busy = true;
+                    }
+                }
+            }
+        }
+        if (!__any(busy)) break;
+        if (busy) {
+            L.step(a, sPAR, sPARi);
+            if (L.term) {
+                L.finish(a, sPAR, sPARi);
+                busy = false;
+            }
+        }
+    }
+}
+
+// Resets the work-queue head on the stream (a kernel, so the sequence can be
+// captured in a graph without a memset node).
+template <int Unused = 0>
+__global__ void reset_queue_kernel(unsigned int* q)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) q[0] = 0u;
+}
+
+} // namespace pt
+} // namespace mpgos

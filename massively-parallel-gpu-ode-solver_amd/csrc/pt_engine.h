@@ -1,0 +1,616 @@
+// Host side of the per-thread module: owns the pinned host mirrors, the device
+// buffers, the HIP stream and events of one solver object and launches the
+// kernels of pt_device.h.  It implements the member functions of the
+// reference's ProblemSolver (SourceCodes/SingleSystem_PerThread_Interface.cuh:
+// constructor 309-518, SolverOption 610-703, SetHost 705-830, Synchronise*
+// 832-965, GetHost 967-1080, Print 1111-1276, Solve 1279-1285, synchronisation
+// 1288-1312) with return codes instead of exit().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "mpgos.h"
+#include "pt_device.h"
+
+namespace mpgos {
+
+void set_error(const std::string& msg);   // thread-local message for mpgos_last_error()
+
+#define MPGOS_HIP_TRY(call)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ::mpgos::set_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " \
+                               + __FILE__ + ":" + std::to_string(__LINE__));             \
+            return MPGOS_E_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+namespace pt {
+
+// Kernel entry points.  WPE is the waves-per-EU hint of the register-budget
+// variant (1 = no constraint); MODE 0 = plain, 1 = persistent.
+template <class Sys, class P, bool KAHAN, int ALG, int MODE, int WPE>
+__global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE)))
+solve_kernel(Args<Sys, P, typename Lane<Sys, P, KAHAN, ALG>::CP> a)
+{
+    if (MODE == MPGOS_KERNEL_PLAIN) plain_body<Sys, P, KAHAN, ALG>(a);
+    else persistent_body<Sys, P, KAHAN, ALG>(a);
+}
+
+// Virtual interface the C ABI talks to; one implementation per (system, precision).
+class SolverBase {
+public:
+    virtual ~SolverBase() {}
+    virtual int init(int algorithm, int nt, int ndo, int device) = 0;
+    virtual int dims(mpgos_pt_dims* d) const = 0;
+    virtual int option(int opt, int idx, double v) = 0;
+    virtual int set_host(int var, int problem, int serial, int step, double v) = 0;
+    virtual int get_host(int var, int problem, int serial, int step, double* v) const = 0;
+    virtual void* host_buffer(int var, size_t* nbytes) = 0;
+    virtual void* device_buffer(int var, size_t* nbytes) = 0;
+    virtual int sync(int var, bool to_device) = 0;
+    virtual int solve() = 0;
+    virtual int synchronise_device() = 0;
+    virtual int insert_sync_point() = 0;
+    virtual int synchronise_solver() = 0;
+    virtual int print(int var, int tid, const char* dir) = 0;
+    virtual int last_solve_ms(double* ms) const = 0;
+    virtual void* stream() = 0;
+    // adjoint (pt_adjoint.h); default: unsupported
+    virtual int adjoint(const void*, void*, void*, int) { set_error("adjoint not supported by this system"); return MPGOS_E_SYSTEM; }
+};
+
+template <class T>
+struct Buffer {
+    T* h = nullptr;   // pinned host mirror
+    T* d = nullptr;   // device
+    size_t n = 0;
+    int alloc(size_t count)
+    {
+        n = count;
+        if (count == 0) return MPGOS_OK;
+        if (hipHostMalloc((void**)&h, count * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+            set_error("failed to allocate pinned host memory");
+            return MPGOS_E_MEMORY;
+        }
+        std::memset(h, 0, count * sizeof(T));
+        if (hipMalloc((void**)&d, count * sizeof(T)) != hipSuccess) {
+            set_error("failed to allocate device memory");
+            return MPGOS_E_MEMORY;
+        }
+        return MPGOS_OK;
+    }
+    void release()
+    {
+        if (h) (void)hipHostFree(h);
+        if (d) (void)hipFree(d);
+        h = nullptr; d = nullptr; n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+template <class Sys, class P, bool KAHAN>
+class Solver : public SolverBase {
+public:
+    using CP = typename std::conditional<KAHAN, double, P>::type;
+    static constexpr int SD = Sys::SD, NCP = Sys::NCP, NSP = Sys::NSP, NISP = Sys::NISP;
+    static constexpr int NE = Sys::NE, NA = Sys::NA, NIA = Sys::NIA;
+
+    int algorithm = MPGOS_RKCK45;
+    int NT = 0, NDO = 0, device = 0;
+    hipStream_t strm = nullptr;
+    hipEvent_t syncEvent = nullptr, evStart = nullptr, evStop = nullptr;
+    int numCU = 0;
+
+    // options (SingleSystem_PerThread_Interface.cuh:340-345, 475-496 defaults)
+    int blockSize = 64, nActive = 0, preferShared = 1;
+    double dt0 = 1e-2, dtmax = 1.0e6, dtmin = 1.0e-12, grow = 5.0, shrink = 0.1, doMinDt = 0.0;
+    int doFreq = 1;
+    std::vector<double> rtol, atol, etol;
+    std::vector<int> edir;
+    int kernelMode = MPGOS_KERNEL_PLAIN, wavesPerSimd = 0, graphCapture = 0, evcap = 0, variant = 0;
+
+    Buffer<P> TD, X, T, cPAR, sPAR, ACC, DOt, DOx, evlog;
+    Buffer<int> sPARi, ACCi, DOidx, status, nev, evlogidx;
+    Buffer<long long> nacc, nrej;
+    unsigned int* queue = nullptr;
+
+    // graph replay
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    std::vector<unsigned char> graphArgs;
+    int graphKey = -1;
+
+    ~Solver() override
+    {
+        if (strm) (void)hipStreamSynchronize(strm);
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+        TD.release(); X.release(); T.release(); cPAR.release(); sPAR.release(); ACC.release();
+        DOt.release(); DOx.release(); evlog.release();
+        sPARi.release(); ACCi.release(); DOidx.release(); status.release(); nev.release(); evlogidx.release();
+        nacc.release(); nrej.release();
+        if (queue) (void)hipFree(queue);
+        if (syncEvent) (void)hipEventDestroy(syncEvent);
+        if (evStart) (void)hipEventDestroy(evStart);
+        if (evStop) (void)hipEventDestroy(evStop);
+        if (strm) (void)hipStreamDestroy(strm);
+    }
+
+    int init(int alg, int nt, int ndo, int dev) override
+    {
+        if (alg != MPGOS_RK4 && alg != MPGOS_RKCK45) { set_error("unknown algorithm"); return MPGOS_E_ARG; }
+        if (nt <= 0 || ndo < 0) { set_error("NT must be positive and NDO non-negative"); return MPGOS_E_ARG; }
+        algorithm = alg; NT = nt; NDO = ndo; device = dev; nActive = nt;
+        MPGOS_HIP_TRY(hipSetDevice(device));
+        MPGOS_HIP_TRY(hipStreamCreateWithFlags(&strm, hipStreamNonBlocking));
+        MPGOS_HIP_TRY(hipEventCreateWithFlags(&syncEvent, hipEventDisableTiming));
+        MPGOS_HIP_TRY(hipEventCreate(&evStart));
+        MPGOS_HIP_TRY(hipEventCreate(&evStop));
+        hipDeviceProp_t prop;
+        MPGOS_HIP_TRY(hipGetDeviceProperties(&prop, device));
+        numCU = prop.multiProcessorCount;
+        blockSize = prop.warpSize;   // reference default: ThreadsPerBlock = warp size (PT-IF:343)
+
+        const size_t nt_ = (size_t)nt;
+        int rc;
+        if ((rc = TD.alloc(2 * nt_)) || (rc = X.alloc(SD * nt_)) || (rc = T.alloc(nt_)) ||
+            (rc = cPAR.alloc(NCP * nt_)) || (rc = sPAR.alloc(NSP)) || (rc = sPARi.alloc(NISP)) ||
+            (rc = ACC.alloc(NA * nt_)) || (rc = ACCi.alloc(NIA * nt_)) || (rc = DOidx.alloc(nt_)) ||
+            (rc = DOt.alloc((size_t)ndo * nt_)) || (rc = DOx.alloc((size_t)SD * ndo * nt_)) ||
+            (rc = status.alloc(nt_)) || (rc = nev.alloc(nt_)) || (rc = nacc.alloc(nt_)) || (rc = nrej.alloc(nt_)))
+            return rc;
+        MPGOS_HIP_TRY(hipMalloc((void**)&queue, 256));
+        rtol.assign(SD, 1e-8);
+        atol.assign(SD, 1e-8);
+        etol.assign(NE, 1e-6);
+        edir.assign(NE, 0);
+        // device buffers start from the (zeroed) host mirrors
+        return sync(MPGOS_PT_ALL, true);
+    }
+
+    int dims(mpgos_pt_dims* d) const override
+    {
+        d->nt = NT; d->sd = SD; d->ncp = NCP; d->nsp = NSP; d->nisp = NISP; d->ne = NE;
+        d->na = NA; d->nia = NIA; d->ndo = NDO; d->algorithm = algorithm;
+        d->precision = KAHAN ? MPGOS_FLOAT_KAHAN : (std::is_same<P, float>::value ? MPGOS_FLOAT : MPGOS_DOUBLE);
+        d->has_vjp = Sys::HAS_VJP ? 1 : 0;
+        return MPGOS_OK;
+    }
+
+    static int bound(const char* fn, const char* what, int v, int lo, int hi)
+    {
+        if (v < lo || v > hi) {
+            char buf[256];
+            if (lo > hi) std::snprintf(buf, sizeof buf, "%s: %s index %d out of range (acceptable index: none)", fn, what, v);
+            else std::snprintf(buf, sizeof buf, "%s: %s index %d out of range (acceptable index: %d-%d)", fn, what, v, lo, hi);
+            set_error(buf);
+            return MPGOS_E_ARG;
+        }
+        return MPGOS_OK;
+    }
+
+    int option(int opt, int idx, double v) override
+    {
+        switch (opt) {
+        case MPGOS_PT_THREADS_PER_BLOCK:
+            if ((int)v < 1 || (int)v > kMaxBlock) { set_error("ThreadsPerBlock must be in 1..256"); return MPGOS_E_ARG; }
+            blockSize = (int)v; return MPGOS_OK;
+        case MPGOS_PT_INITIAL_TIME_STEP: dt0 = v; return MPGOS_OK;
+        case MPGOS_PT_ACTIVE_NUMBER_OF_THREADS:
+            if ((int)v < 0 || (int)v > NT) { set_error("ActiveNumberOfThreads must be in 0..NT"); return MPGOS_E_ARG; }
+            nActive = (int)v; return MPGOS_OK;
+        case MPGOS_PT_MAXIMUM_TIME_STEP: dtmax = v; return MPGOS_OK;
+        case MPGOS_PT_MINIMUM_TIME_STEP: dtmin = v; return MPGOS_OK;
+        case MPGOS_PT_TIME_STEP_GROW_LIMIT: grow = v; return MPGOS_OK;
+        case MPGOS_PT_TIME_STEP_SHRINK_LIMIT: shrink = v; return MPGOS_OK;
+        case MPGOS_PT_RELATIVE_TOLERANCE:
+            if (int rc = bound("SolverOption", "RelativeTolerance", idx, 0, SD - 1)) return rc;
+            rtol[idx] = v; return MPGOS_OK;
+        case MPGOS_PT_ABSOLUTE_TOLERANCE:
+            if (int rc = bound("SolverOption", "AbsoluteTolerance", idx, 0, SD - 1)) return rc;
+            atol[idx] = v; return MPGOS_OK;
+        case MPGOS_PT_EVENT_TOLERANCE:
+            if (int rc = bound("SolverOption", "EventTolerance", idx, 0, NE - 1)) return rc;
+            etol[idx] = v; return MPGOS_OK;
+        case MPGOS_PT_EVENT_DIRECTION:
+            if (int rc = bound("SolverOption", "EventDirection", idx, 0, NE - 1)) return rc;
+            edir[idx] = (int)v; return MPGOS_OK;
+        case MPGOS_PT_DENSE_OUTPUT_MINIMUM_TIME_STEP: doMinDt = v; return MPGOS_OK;
+        case MPGOS_PT_DENSE_OUTPUT_SAVE_FREQUENCY: doFreq = (int)v; return MPGOS_OK;
+        case MPGOS_PT_PREFER_SHARED_MEMORY: preferShared = (int)v != 0; return MPGOS_OK;
+        case MPGOS_PT_KERNEL_MODE:
+            if ((int)v != MPGOS_KERNEL_PLAIN && (int)v != MPGOS_KERNEL_PERSISTENT) { set_error("KernelMode must be 0 or 1"); return MPGOS_E_ARG; }
+            kernelMode = (int)v; return MPGOS_OK;
+        case MPGOS_PT_PERSISTENT_WAVES_PER_SIMD:
+            if ((int)v < 0 || (int)v > 8) { set_error("PersistentWavesPerSimd must be in 0..8"); return MPGOS_E_ARG; }
+            wavesPerSimd = (int)v; return MPGOS_OK;
+        case MPGOS_PT_GRAPH_CAPTURE: graphCapture = (int)v != 0; return MPGOS_OK;
+        case MPGOS_PT_LAUNCH_VARIANT:
+            if ((int)v < 0 || (int)v > 3) { set_error("LaunchVariant must be in 0..3"); return MPGOS_E_ARG; }
+            variant = (int)v; return MPGOS_OK;
+        case MPGOS_PT_EVENT_LOG_CAPACITY: {
+            if ((int)v < 0) { set_error("EventLogCapacity must be >= 0"); return MPGOS_E_ARG; }
+            MPGOS_HIP_TRY(hipStreamSynchronize(strm));
+            evlog.release(); evlogidx.release();
+            evcap = (int)v;
+            int rc;
+            if ((rc = evlog.alloc((size_t)(SD + 1) * evcap * NT)) || (rc = evlogidx.alloc((size_t)evcap * NT))) return rc;
+            // start from the zeroed host mirrors so unused slots read as zero
+            if ((rc = copy(evlog, true)) || (rc = copy(evlogidx, true))) return rc;
+            return MPGOS_OK;
+        }
+        default:
+            set_error("SolverOption: unknown or not applicable option " + std::to_string(opt));
+            return MPGOS_E_ARG;
+        }
+    }
+
+    // (variable, problem, serial, step) -> pointer into the host mirror; exact
+    // index rules and bound checks of SetHost/GetHost (PT-IF:705-830, 967-1080).
+    template <class F>
+    int access(const char* fn, int var, int problem, int serial, int step, F&& f) const
+    {
+        int rc;
+        switch (var) {
+        case MPGOS_PT_TIME_DOMAIN: case MPGOS_PT_ACTUAL_STATE: case MPGOS_PT_CONTROL_PARAMETERS:
+        case MPGOS_PT_ACCESSORIES: case MPGOS_PT_INTEGER_ACCESSORIES: case MPGOS_PT_DENSE_TIME: {
+            if ((rc = bound(fn, "ProblemNumber", problem, 0, NT - 1))) return rc;
+            const size_t idx = (size_t)problem + (size_t)serial * NT;
+            switch (var) {
+            case MPGOS_PT_TIME_DOMAIN: if ((rc = bound(fn, "TimeDomain", serial, 0, 1))) return rc; f(&TD.h[idx]); break;
+            case MPGOS_PT_ACTUAL_STATE: if ((rc = bound(fn, "ActualState", serial, 0, SD - 1))) return rc; f(&X.h[idx]); break;
+            case MPGOS_PT_CONTROL_PARAMETERS: if ((rc = bound(fn, "ControlParameters", serial, 0, NCP - 1))) return rc; f(&cPAR.h[idx]); break;
+            case MPGOS_PT_ACCESSORIES: if ((rc = bound(fn, "Accessories", serial, 0, NA - 1))) return rc; f(&ACC.h[idx]); break;
+            case MPGOS_PT_INTEGER_ACCESSORIES: if ((rc = bound(fn, "IntegerAccessories", serial, 0, NIA - 1))) return rc; f(&ACCi.h[idx]); break;
+            default: if ((rc = bound(fn, "DenseTime", serial, 0, NDO - 1))) return rc; f(&DOt.h[idx]); break;
+            }
+            return MPGOS_OK;
+        }
+        case MPGOS_PT_ACTUAL_TIME: case MPGOS_PT_DENSE_INDEX:
+        case MPGOS_PT_ACCEPTED_STEPS: case MPGOS_PT_REJECTED_STEPS: case MPGOS_PT_STATUS: case MPGOS_PT_EVENT_COUNT:
+            if ((rc = bound(fn, "ProblemNumber", problem, 0, NT - 1))) return rc;
+            switch (var) {
+            case MPGOS_PT_ACTUAL_TIME: f(&T.h[problem]); break;
+            case MPGOS_PT_DENSE_INDEX: f(&DOidx.h[problem]); break;
+            case MPGOS_PT_ACCEPTED_STEPS: f(&nacc.h[problem]); break;
+            case MPGOS_PT_REJECTED_STEPS: f(&nrej.h[problem]); break;
+            case MPGOS_PT_STATUS: f(&status.h[problem]); break;
+            default: f(&nev.h[problem]); break;
+            }
+            return MPGOS_OK;
+        case MPGOS_PT_SHARED_PARAMETERS:
+            if ((rc = bound(fn, "SharedParameters", serial, 0, NSP - 1))) return rc;
+            f(&sPAR.h[serial]); return MPGOS_OK;
+        case MPGOS_PT_INTEGER_SHARED_PARAMETERS:
+            if ((rc = bound(fn, "IntegerSharedParameters", serial, 0, NISP - 1))) return rc;
+            f(&sPARi.h[serial]); return MPGOS_OK;
+        case MPGOS_PT_DENSE_STATE: {
+            if ((rc = bound(fn, "ProblemNumber", problem, 0, NT - 1))) return rc;
+            if ((rc = bound(fn, "DenseState/ComponentNumber", serial, 0, SD - 1))) return rc;
+            if ((rc = bound(fn, "DenseState/TimeStepNumber", step, 0, NDO - 1))) return rc;
+            f(&DOx.h[(size_t)problem + (size_t)serial * NT + (size_t)step * NT * SD]);
+            return MPGOS_OK;
+        }
+        case MPGOS_PT_EVENT_LOG: {
+            if ((rc = bound(fn, "ProblemNumber", problem, 0, NT - 1))) return rc;
+            if ((rc = bound(fn, "EventLog/Column", serial, 0, SD))) return rc;
+            if ((rc = bound(fn, "EventLog/Hit", step, 0, evcap - 1))) return rc;
+            f(&evlog.h[(size_t)problem + ((size_t)step * (SD + 1) + serial) * NT]);
+            return MPGOS_OK;
+        }
+        case MPGOS_PT_EVENT_LOG_INDEX: {
+            if ((rc = bound(fn, "ProblemNumber", problem, 0, NT - 1))) return rc;
+            if ((rc = bound(fn, "EventLogIndex/Hit", step, 0, evcap - 1))) return rc;
+            f(&evlogidx.h[(size_t)problem + (size_t)step * NT]);
+            return MPGOS_OK;
+        }
+        default:
+            set_error(std::string(fn) + ": variable not applicable with this argument configuration");
+            return MPGOS_E_ARG;
+        }
+    }
+
+    int set_host(int var, int problem, int serial, int step, double v) override
+    {
+        if (var == MPGOS_PT_DENSE_INDEX) {
+            // PT-IF:767-770: the value itself is bound-checked
+            if (int rc = bound("SetHost", "DenseIndex-Value", (int)v, 0, NDO - 1)) return rc;
+        }
+        if (var >= MPGOS_PT_ACCEPTED_STEPS) { set_error("SetHost: output-only variable"); return MPGOS_E_ARG; }
+        return access("SetHost", var, problem, serial, step, [&](auto* p) {
+            using T_ = typename std::remove_pointer<decltype(p)>::type;
+            *p = std::is_integral<T_>::value ? (T_)(int)v : (T_)v;
+        });
+    }
+
+    int get_host(int var, int problem, int serial, int step, double* v) const override
+    {
+        return access("GetHost", var, problem, serial, step, [&](auto* p) { *v = (double)*p; });
+    }
+
+    template <class T_> static void* ret(Buffer<T_>& b, bool host, size_t* nbytes)
+    {
+        if (nbytes) *nbytes = b.bytes();
+        return host ? (void*)b.h : (void*)b.d;
+    }
+
+    void* buffer(int var, bool host, size_t* nbytes)
+    {
+        switch (var) {
+        case MPGOS_PT_TIME_DOMAIN: return ret(TD, host, nbytes);
+        case MPGOS_PT_ACTUAL_STATE: return ret(X, host, nbytes);
+        case MPGOS_PT_ACTUAL_TIME: return ret(T, host, nbytes);
+        case MPGOS_PT_CONTROL_PARAMETERS: return ret(cPAR, host, nbytes);
+        case MPGOS_PT_SHARED_PARAMETERS: return ret(sPAR, host, nbytes);
+        case MPGOS_PT_INTEGER_SHARED_PARAMETERS: return ret(sPARi, host, nbytes);
+        case MPGOS_PT_ACCESSORIES: return ret(ACC, host, nbytes);
+        case MPGOS_PT_INTEGER_ACCESSORIES: return ret(ACCi, host, nbytes);
+        case MPGOS_PT_DENSE_INDEX: return ret(DOidx, host, nbytes);
+        case MPGOS_PT_DENSE_TIME: return ret(DOt, host, nbytes);
+        case MPGOS_PT_DENSE_STATE: return ret(DOx, host, nbytes);
+        case MPGOS_PT_ACCEPTED_STEPS: return ret(nacc, host, nbytes);
+        case MPGOS_PT_REJECTED_STEPS: return ret(nrej, host, nbytes);
+        case MPGOS_PT_STATUS: return ret(status, host, nbytes);
+        case MPGOS_PT_EVENT_COUNT: return ret(nev, host, nbytes);
+        case MPGOS_PT_EVENT_LOG: return ret(evlog, host, nbytes);
+        case MPGOS_PT_EVENT_LOG_INDEX: return ret(evlogidx, host, nbytes);
+        default:
+            set_error("buffer: variable has no single buffer");
+            if (nbytes) *nbytes = 0;
+            return nullptr;
+        }
+    }
+    void* host_buffer(int var, size_t* nbytes) override { return buffer(var, true, nbytes); }
+    void* device_buffer(int var, size_t* nbytes) override { return buffer(var, false, nbytes); }
+
+    template <class T_> int copy(Buffer<T_>& b, bool to_device)
+    {
+        if (b.n == 0) return MPGOS_OK;
+        if (to_device) MPGOS_HIP_TRY(hipMemcpyAsync(b.d, b.h, b.bytes(), hipMemcpyHostToDevice, strm));
+        else MPGOS_HIP_TRY(hipMemcpyAsync(b.h, b.d, b.bytes(), hipMemcpyDeviceToHost, strm));
+        return MPGOS_OK;
+    }
+
+    // PT-IF:832-965
+    int sync(int var, bool to_device) override
+    {
+        MPGOS_HIP_TRY(hipSetDevice(device));
+        int rc = MPGOS_OK;
+        switch (var) {
+        case MPGOS_PT_TIME_DOMAIN: return copy(TD, to_device);
+        case MPGOS_PT_ACTUAL_STATE: return copy(X, to_device);
+        case MPGOS_PT_ACTUAL_TIME: return copy(T, to_device);
+        case MPGOS_PT_CONTROL_PARAMETERS: return copy(cPAR, to_device);
+        case MPGOS_PT_SHARED_PARAMETERS: return copy(sPAR, to_device);
+        case MPGOS_PT_INTEGER_SHARED_PARAMETERS: return copy(sPARi, to_device);
+        case MPGOS_PT_ACCESSORIES: return copy(ACC, to_device);
+        case MPGOS_PT_INTEGER_ACCESSORIES: return copy(ACCi, to_device);
+        case MPGOS_PT_DENSE_OUTPUT:
+            if ((rc = copy(DOidx, to_device)) || (rc = copy(DOt, to_device)) || (rc = copy(DOx, to_device))) return rc;
+            return MPGOS_OK;
+        case MPGOS_PT_ALL:
+            if ((rc = copy(TD, to_device)) || (rc = copy(X, to_device)) || (rc = copy(T, to_device)) ||
+                (rc = copy(cPAR, to_device)) || (rc = copy(sPAR, to_device)) || (rc = copy(ACC, to_device)) ||
+                (rc = copy(sPARi, to_device)) || (rc = copy(ACCi, to_device)) || (rc = copy(DOidx, to_device)) ||
+                (rc = copy(DOt, to_device)) || (rc = copy(DOx, to_device)))
+                return rc;
+            if (!to_device) {
+                if ((rc = copy(nacc, false)) || (rc = copy(nrej, false)) || (rc = copy(status, false)) || (rc = copy(nev, false))) return rc;
+            }
+            return MPGOS_OK;
+        case MPGOS_PT_ACCEPTED_STEPS: case MPGOS_PT_REJECTED_STEPS: case MPGOS_PT_STATUS: case MPGOS_PT_EVENT_COUNT:
+        case MPGOS_PT_EVENT_LOG: case MPGOS_PT_EVENT_LOG_INDEX:
+            if (to_device) { set_error("SynchroniseFromHostToDevice: output-only variable"); return MPGOS_E_ARG; }
+            switch (var) {
+            case MPGOS_PT_ACCEPTED_STEPS: return copy(nacc, false);
+            case MPGOS_PT_REJECTED_STEPS: return copy(nrej, false);
+            case MPGOS_PT_STATUS: return copy(status, false);
+            case MPGOS_PT_EVENT_COUNT: return copy(nev, false);
+            case MPGOS_PT_EVENT_LOG: return copy(evlog, false);
+            default: return copy(evlogidx, false);
+            }
+        default:
+            set_error("Synchronise: variable not applicable");
+            return MPGOS_E_ARG;
+        }
+    }
+
+    Args<Sys, P, CP> make_args() const
+    {
+        Args<Sys, P, CP> a;
+        std::memset(&a, 0, sizeof a);
+        a.TD = TD.d; a.X = X.d; a.T = T.d; a.cPAR = cPAR.d; a.sPAR = sPAR.d; a.sPARi = sPARi.d;
+        a.ACC = ACC.d; a.ACCi = ACCi.d; a.DOidx = DOidx.d; a.DOt = DOt.d; a.DOx = DOx.d;
+        a.nacc = nacc.d; a.nrej = nrej.d; a.status = status.d; a.nev = nev.d;
+        a.evlog = evlog.d; a.evlogidx = evlogidx.d; a.queue = queue;
+        a.NT = NT; a.nActive = nActive; a.NDO = NDO; a.evcap = evcap; a.preferShared = preferShared;
+        a.doFreq = doFreq;
+        a.dt0 = (CP)(P)dt0;   // options are stored as Precision in the reference (PT-IF:619-650)
+        a.dtmax = (CP)(P)dtmax; a.dtmin = (CP)(P)dtmin; a.grow = (CP)(P)grow; a.shrink = (CP)(P)shrink;
+        if (KAHAN) { a.dt0 = (CP)dt0; a.dtmax = (CP)dtmax; a.dtmin = (CP)dtmin; a.grow = (CP)grow; a.shrink = (CP)shrink; }
+        a.doMinDt = (P)doMinDt;
+        for (int i = 0; i < SD; i++) {
+            a.rtol[i] = KAHAN ? (CP)rtol[i] : (CP)(P)rtol[i];
+            a.atol[i] = KAHAN ? (CP)atol[i] : (CP)(P)atol[i];
+        }
+        for (int i = 0; i < NE; i++) { a.etol[i] = (P)etol[i]; a.edir[i] = edir[i]; }
+        return a;
+    }
+
+    using KernelFn = void (*)(Args<Sys, P, CP>);
+
+    template <int ALG, int MODE>
+    static KernelFn pick_variant(int v)
+    {
+        switch (v) {
+        case 1: return solve_kernel<Sys, P, KAHAN, ALG, MODE, 2>;
+        case 2: return solve_kernel<Sys, P, KAHAN, ALG, MODE, 4>;
+        case 3: return solve_kernel<Sys, P, KAHAN, ALG, MODE, 8>;
+        default: return solve_kernel<Sys, P, KAHAN, ALG, MODE, 1>;
+        }
+    }
+
+    KernelFn kernel() const
+    {
+        if (algorithm == MPGOS_RK4)
+            return kernelMode == MPGOS_KERNEL_PLAIN ? pick_variant<MPGOS_RK4, MPGOS_KERNEL_PLAIN>(variant)
+                                                    : pick_variant<MPGOS_RK4, MPGOS_KERNEL_PERSISTENT>(variant);
+        return kernelMode == MPGOS_KERNEL_PLAIN ? pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PLAIN>(variant)
+                                                : pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PERSISTENT>(variant);
+    }
+
+    size_t dyn_lds() const { return preferShared ? (size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int) : 0; }
+
+    int launch_geometry(KernelFn fn, int& grid) const
+    {
+        if (kernelMode == MPGOS_KERNEL_PLAIN) {
+            grid = NT / blockSize + (NT % blockSize == 0 ? 0 : 1);   // PT-IF:344
+            return MPGOS_OK;
+        }
+        int perCU = 0;
+        MPGOS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)fn, blockSize, dyn_lds()));
+        if (perCU < 1) perCU = 1;
+        const int wavesPerBlock = (blockSize + 63) / 64;
+        if (wavesPerSimd > 0) {
+            const int want = (4 * wavesPerSimd + wavesPerBlock - 1) / wavesPerBlock;
+            if (want < perCU) perCU = want;
+        }
+        grid = perCU * numCU;
+        const int needed = nActive / blockSize + (nActive % blockSize == 0 ? 0 : 1);
+        if (grid > needed) grid = needed;
+        if (grid < 1) grid = 1;
+        return MPGOS_OK;
+    }
+
+    int enqueue(const Args<Sys, P, CP>& a, KernelFn fn, int grid)
+    {
+        if (kernelMode == MPGOS_KERNEL_PERSISTENT)
+            hipLaunchKernelGGL(reset_queue_kernel<0>, dim3(1), dim3(64), 0, strm, queue);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(blockSize), dyn_lds(), strm, a);
+        MPGOS_HIP_TRY(hipGetLastError());
+        return MPGOS_OK;
+    }
+
+    // PT-IF:1279-1285
+    int solve() override
+    {
+        MPGOS_HIP_TRY(hipSetDevice(device));
+        if (nActive == 0) return MPGOS_OK;
+        const Args<Sys, P, CP> a = make_args();
+        KernelFn fn = kernel();
+        int grid = 0;
+        if (int rc = launch_geometry(fn, grid)) return rc;
+        MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+        if (!graphCapture) {
+            if (int rc = enqueue(a, fn, grid)) return rc;
+        } else {
+            const int key = (algorithm << 8) | (kernelMode << 4) | variant;
+            const unsigned char* ab = reinterpret_cast<const unsigned char*>(&a);
+            const bool same = graphExec && key == graphKey && graphArgs.size() == sizeof a &&
+                              std::memcmp(graphArgs.data(), ab, sizeof a) == 0 && graphGrid == grid &&
+                              graphBlock == blockSize;
+            if (!same) {
+                if (graphExec) { (void)hipGraphExecDestroy(graphExec); graphExec = nullptr; }
+                if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+                MPGOS_HIP_TRY(hipStreamBeginCapture(strm, hipStreamCaptureModeThreadLocal));
+                int rc = enqueue(a, fn, grid);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(strm, &g);
+                if (rc) return rc;
+                if (e != hipSuccess) { set_error(std::string("graph capture failed: ") + hipGetErrorString(e)); return MPGOS_E_HIP; }
+                graph = g;
+                MPGOS_HIP_TRY(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
+                graphArgs.assign(ab, ab + sizeof a);
+                graphKey = key; graphGrid = grid; graphBlock = blockSize;
+            }
+            MPGOS_HIP_TRY(hipGraphLaunch(graphExec, strm));
+        }
+        MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
+        return MPGOS_OK;
+    }
+    int graphGrid = 0, graphBlock = 0;
+
+    int synchronise_device() override { MPGOS_HIP_TRY(hipSetDevice(device)); MPGOS_HIP_TRY(hipDeviceSynchronize()); return MPGOS_OK; }
+    int insert_sync_point() override { MPGOS_HIP_TRY(hipSetDevice(device)); MPGOS_HIP_TRY(hipEventRecord(syncEvent, strm)); return MPGOS_OK; }
+    int synchronise_solver() override { MPGOS_HIP_TRY(hipSetDevice(device)); MPGOS_HIP_TRY(hipEventSynchronize(syncEvent)); return MPGOS_OK; }
+    int last_solve_ms(double* ms) const override
+    {
+        float f = 0.f;
+        MPGOS_HIP_TRY(hipEventElapsedTime(&f, evStart, evStop));
+        *ms = f;
+        return MPGOS_OK;
+    }
+    void* stream() override { return (void*)strm; }
+
+    // ------------------------------------------------------------ Print (PT-IF:1082-1276)
+    template <class T_>
+    static void write_general(const std::string& file, int rows, int cols, const T_* data)
+    {
+        std::ofstream f(file);
+        f.precision(10);
+        f.flags(std::ios::scientific);
+        for (int i = 0; i < rows; i++) {
+            for (int j = 0; j < cols; j++) {
+                f.width(18);
+                f << data[i + (size_t)j * rows];
+                if (j < cols - 1) f << ',';
+            }
+            f << '\n';
+        }
+    }
+
+    int print(int var, int tid, const char* dir) override
+    {
+        const std::string pre = dir ? std::string(dir) + "/" : std::string();
+        switch (var) {
+        case MPGOS_PT_TIME_DOMAIN: write_general(pre + "TimeDomainInSolverObject.txt", NT, 2, TD.h); return MPGOS_OK;
+        case MPGOS_PT_ACTUAL_STATE: write_general(pre + "ActualStateInSolverObject.txt", NT, SD, X.h); return MPGOS_OK;
+        case MPGOS_PT_ACTUAL_TIME: write_general(pre + "ActualTimeInSolverObject.txt", NT, 1, T.h); return MPGOS_OK;
+        case MPGOS_PT_CONTROL_PARAMETERS: write_general(pre + "ControlParametersInSolverObject.txt", NT, NCP, cPAR.h); return MPGOS_OK;
+        case MPGOS_PT_SHARED_PARAMETERS: write_general(pre + "SharedParametersInSolverObject.txt", NSP, 1, sPAR.h); return MPGOS_OK;
+        case MPGOS_PT_INTEGER_SHARED_PARAMETERS: write_general(pre + "IntegerSharedParametersInSolverObject.txt", NISP, 1, sPARi.h); return MPGOS_OK;
+        case MPGOS_PT_ACCESSORIES: write_general(pre + "AccessoriesInSolverObject.txt", NT, NA, ACC.h); return MPGOS_OK;
+        case MPGOS_PT_INTEGER_ACCESSORIES: write_general(pre + "IntegerAccessoriesInSolverObject.txt", NT, NIA, ACCi.h); return MPGOS_OK;
+        case MPGOS_PT_DENSE_OUTPUT: break;
+        default: set_error("Print: variable not applicable"); return MPGOS_E_ARG;
+        }
+        if (int rc = bound("Print", "Thread", tid, 0, NT - 1)) return rc;
+        std::ofstream f(pre + "DenseOutput_" + std::to_string(tid) + ".txt");
+        f.precision(10);
+        f.flags(std::ios::scientific);
+        auto row = [&](const char* title, int n, auto get) {
+            f << title << ":\n";
+            for (int i = 0; i < n; i++) { f.width(18); f << get(i); if (i < n - 1) f << ','; }
+        };
+        row("ControlParameters", NCP, [&](int i) { return cPAR.h[tid + (size_t)i * NT]; }); f << '\n';
+        row("SharedParameters", NSP, [&](int i) { return sPAR.h[i]; }); f << '\n';
+        row("IntegerSharedParameters", NISP, [&](int i) { return sPARi.h[i]; }); f << '\n';
+        row("Accessories", NA, [&](int i) { return ACC.h[tid + (size_t)i * NT]; }); f << '\n';
+        row("IntegerAccessories", NIA, [&](int i) { return ACCi.h[tid + (size_t)i * NT]; }); f << "\n\n";
+        f << "Time series:\n";
+        if (NDO > 0) {
+            const int n = DOidx.h[tid] < NDO ? DOidx.h[tid] : NDO;
+            for (int i = 0; i < n; i++) {
+                f.width(18); f << DOt.h[tid + (size_t)i * NT] << ',';
+                for (int j = 0; j < SD; j++) {
+                    f.width(18); f << DOx.h[tid + (size_t)j * NT + (size_t)i * NT * SD];
+                    if (j < SD - 1) f << ',';
+                }
+                f << '\n';
+            }
+        }
+        return MPGOS_OK;
+    }
+};
+
+} // namespace pt
+} // namespace mpgos

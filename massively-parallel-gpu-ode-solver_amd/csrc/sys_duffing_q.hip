@@ -1,0 +1,10 @@
+// Kernel instantiations of the registered system "duffing_q" (all precisions,
+// algorithms, kernel forms and register-budget variants).
+#include "registry.h"
+#include "systems/duffing.h"
+
+namespace mpgos {
+namespace pt {
+MPGOS_PT_FACTORY(duffing_q, ::mpgos::systems::DuffingQ)
+} // namespace pt
+} // namespace mpgos

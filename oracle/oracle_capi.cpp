@@ -1,0 +1,93 @@
+// ORACLE -- test infrastructure only (see pt_oracle.h).  C entry points for
+// the Python tests and bench.py's cpu_baseline leg (ctypes).
+#include <omp.h>
+
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "flops.h"
+#include "pt_oracle.h"
+#include "systems/affine2.h"
+#include "systems/duffing.h"
+#include "systems/duffing_dw.h"
+#include "systems/lorenz.h"
+
+namespace oracle {
+namespace flops {
+thread_local long long g_count = 0;
+F cos(F a) { g_count += 38; return F(mpgos::math::cos(a.v)); }
+F sin(F a) { g_count += 38; return F(mpgos::math::sin(a.v)); }
+F exp(F a) { g_count += 31; return F(mpgos::math::exp(a.v)); }
+F log(F a) { g_count += 28; return F(mpgos::math::log(a.v)); }
+F pow(F a, F b) { g_count += 60; return F(mpgos::math::pow(a.v, b.v)); }
+} // namespace flops
+
+template <class Sys, class S>
+static void run_all(const Job& j)
+{
+    std::vector<S> sp(Sys::NSP > 0 ? Sys::NSP : 1);
+    std::vector<int> spi(Sys::NISP > 0 ? Sys::NISP : 1);
+    for (int i = 0; i < Sys::NSP; i++) sp[i] = S(j.sPAR[i]);
+    for (int i = 0; i < Sys::NISP; i++) spi[i] = j.sPARi[i];
+    const int threads = j.nthreads > 0 ? j.nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+    for (int tid = 0; tid < j.n_active; tid++) {
+        std::vector<S> spl(sp);   // per-thread copy: callbacks may not write it, but keep lanes independent
+        std::vector<int> spil(spi);
+        Instance<Sys, S> inst(j, tid, spl.data(), spil.data());
+        inst.run();
+    }
+}
+
+typedef void (*Runner)(const Job&);
+struct Entry { const char* name; Runner run; Runner count; };
+
+static const Entry kSystems[] = {
+    {"duffing", run_all<mpgos::systems::Duffing, double>, run_all<mpgos::systems::Duffing, flops::F>},
+    {"duffing_q", run_all<mpgos::systems::DuffingQ, double>, run_all<mpgos::systems::DuffingQ, flops::F>},
+    {"duffing_eps", run_all<mpgos::systems::DuffingEps, double>, run_all<mpgos::systems::DuffingEps, flops::F>},
+    {"duffing_dw", run_all<mpgos::systems::DuffingDW, double>, run_all<mpgos::systems::DuffingDW, flops::F>},
+    {"lorenz", run_all<mpgos::systems::Lorenz, double>, run_all<mpgos::systems::Lorenz, flops::F>},
+    {"affine2", run_all<mpgos::systems::Affine2, double>, run_all<mpgos::systems::Affine2, flops::F>},
+};
+
+static const Entry* find(const char* name)
+{
+    for (const auto& e : kSystems)
+        if (std::strcmp(e.name, name) == 0) return &e;
+    return nullptr;
+}
+
+} // namespace oracle
+
+extern "C" {
+
+int oracle_pt_solve(const char* system, oracle::Job* job, double* seconds)
+{
+    const oracle::Entry* e = oracle::find(system);
+    if (!e || !job) return -4;
+    const auto t0 = std::chrono::steady_clock::now();
+    e->run(*job);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return 0;
+}
+
+// Algorithmic FLOPs of the same integration (single thread).
+int oracle_pt_count_flops(const char* system, oracle::Job* job, long long* flops)
+{
+    const oracle::Entry* e = oracle::find(system);
+    if (!e || !job || !flops) return -4;
+    oracle::Job j = *job;
+    j.nthreads = 1;
+    oracle::flops::g_count = 0;
+    e->count(j);
+    *flops = oracle::flops::g_count;
+    return 0;
+}
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+} // extern "C"

@@ -1,0 +1,127 @@
+"""ORACLE -- test infrastructure only (tests/, __graft_entry__.smoke(),
+bench.py's cpu_baseline).  ctypes front end of oracle/liboracle.so, the FP64
+OpenMP restatement of the reference's per-thread solver (pt_oracle.h).
+
+`run(system, spec)` integrates the instances described by `spec` (a dict of
+numpy arrays in the reference's SoA layout plus solver options, see
+PTSpec in mpgos/workloads.py) and returns the outputs as numpy arrays.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_longlong)
+
+
+class Job(ctypes.Structure):
+    _fields_ = [
+        ("nt", ctypes.c_int), ("ndo", ctypes.c_int), ("algorithm", ctypes.c_int), ("n_active", ctypes.c_int),
+        ("dt0", ctypes.c_double), ("dtmax", ctypes.c_double), ("dtmin", ctypes.c_double),
+        ("grow", ctypes.c_double), ("shrink", ctypes.c_double), ("do_min_dt", ctypes.c_double),
+        ("do_freq", ctypes.c_int),
+        ("rtol", _dp), ("atol", _dp), ("etol", _dp), ("edir", _ip),
+        ("evcap", ctypes.c_int),
+        ("TD", _dp), ("X", _dp), ("T", _dp), ("cPAR", _dp), ("sPAR", _dp), ("sPARi", _ip),
+        ("ACC", _dp), ("ACCi", _ip), ("DOidx", _ip), ("DOt", _dp), ("DOx", _dp),
+        ("nacc", _lp), ("nrej", _lp), ("status", _ip), ("nev", _ip), ("evlog", _dp), ("evlogidx", _ip),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.oracle_pt_solve.restype = ctypes.c_int
+        L.oracle_pt_solve.argtypes = [ctypes.c_char_p, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_double)]
+        L.oracle_pt_count_flops.restype = ctypes.c_int
+        L.oracle_pt_count_flops.argtypes = [ctypes.c_char_p, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_longlong)]
+        L.oracle_max_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a, typ):
+    return a.ctypes.data_as(typ) if a is not None and a.size else ctypes.cast(None, typ)
+
+
+def run(system, spec, nthreads=0, count_flops=False):
+    """Integrate with the oracle.  `spec` keys: algorithm, nt, ndo, n_active,
+    options (dict), rtol, atol, etol, edir, evcap, and the SoA arrays TD, X, T,
+    cPAR, sPAR, sPARi, ACC, ACCi, DOidx, DOt, DOx (float64 / int32).  Returns a
+    dict of output arrays (copies) plus 'seconds' (or 'flops')."""
+    nt = int(spec["nt"])
+    ndo = int(spec.get("ndo", 0))
+    evcap = int(spec.get("evcap", 0))
+    o = dict(dt0=1e-2, dtmax=1e6, dtmin=1e-12, grow=5.0, shrink=0.1, do_min_dt=0.0, do_freq=1)
+    o.update(spec.get("options", {}))
+
+    def f64(name, shape):
+        a = spec.get(name)
+        a = np.zeros(shape, np.float64) if a is None else np.array(a, dtype=np.float64, copy=True)
+        return np.ascontiguousarray(a.reshape(shape))
+
+    def i32(name, shape):
+        a = spec.get(name)
+        a = np.zeros(shape, np.int32) if a is None else np.array(a, dtype=np.int32, copy=True)
+        return np.ascontiguousarray(a.reshape(shape))
+
+    sd, ncp, na, nia, nsp, nisp, ne = (spec[k] for k in ("sd", "ncp", "na", "nia", "nsp", "nisp", "ne"))
+    out = {
+        "TD": f64("TD", (2, nt)), "X": f64("X", (sd, nt)), "T": f64("T", (nt,)), "cPAR": f64("cPAR", (ncp, nt)),
+        "sPAR": f64("sPAR", (nsp,)), "sPARi": i32("sPARi", (nisp,)), "ACC": f64("ACC", (na, nt)),
+        "ACCi": i32("ACCi", (nia, nt)), "DOidx": i32("DOidx", (nt,)), "DOt": f64("DOt", (ndo, nt)),
+        "DOx": f64("DOx", (ndo, sd, nt)),
+        "nacc": np.zeros(nt, np.int64), "nrej": np.zeros(nt, np.int64), "status": np.zeros(nt, np.int32),
+        "nev": np.zeros(nt, np.int32), "evlog": np.zeros((evcap, sd + 1, nt), np.float64),
+        "evlogidx": np.zeros((evcap, nt), np.int32),
+    }
+    rtol = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("rtol", 1e-8), np.float64), (sd,)))
+    atol = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("atol", 1e-8), np.float64), (sd,)))
+    etol = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("etol", 1e-6), np.float64), (max(ne, 1),)))
+    edir = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("edir", 0), np.int32), (max(ne, 1),)))
+    j = Job()
+    j.nt, j.ndo, j.algorithm = nt, ndo, int(spec["algorithm"])
+    j.n_active = int(spec.get("n_active", nt))
+    for k in ("dt0", "dtmax", "dtmin", "grow", "shrink", "do_min_dt"):
+        setattr(j, k, float(o[k]))
+    j.do_freq = int(o["do_freq"])
+    j.rtol, j.atol, j.etol, j.edir = _ptr(rtol, _dp), _ptr(atol, _dp), _ptr(etol, _dp), _ptr(edir, _ip)
+    j.evcap = evcap
+    for k in ("TD", "X", "T", "cPAR", "sPAR", "ACC", "DOt", "DOx", "evlog"):
+        setattr(j, k, _ptr(out[k], _dp))
+    for k in ("sPARi", "ACCi", "DOidx", "status", "nev", "evlogidx"):
+        setattr(j, k, _ptr(out[k], _ip))
+    j.nacc, j.nrej = _ptr(out["nacc"], _lp), _ptr(out["nrej"], _lp)
+    j.nthreads = int(nthreads)
+    if count_flops:
+        fl = ctypes.c_longlong()
+        rc = lib().oracle_pt_count_flops(system.encode(), ctypes.byref(j), ctypes.byref(fl))
+        out["flops"] = fl.value
+    else:
+        sec = ctypes.c_double()
+        rc = lib().oracle_pt_solve(system.encode(), ctypes.byref(j), ctypes.byref(sec))
+        out["seconds"] = sec.value
+    if rc != 0:
+        raise RuntimeError(f"oracle failed for system {system!r} (rc={rc})")
+    return out
+
+
+def max_threads():
+    return lib().oracle_max_threads()
