@@ -20,8 +20,8 @@
 //  * log: x = 2^k m, m in [sqrt(2)/2, sqrt(2)), f = m-1, s = f/(2+f),
 //    log(1+f) = f - (f^2/2 - s (f^2/2 + R(s^2))), the 2 atanh(s) form of
 //    Tang / Sun fdlibm, with R fitted here.
-//  * pow(x, y) = exp(y log x) for x > 0 (a few ulp for |y log x| < ~50, which
-//    covers the controller's x^(1/5) and x^(1/4)).
+//  * pow(x, y) = exp(y log x) with log x carried in double-double, a few ulp
+//    over the controller's x^(1/5) and x^(1/4) for x up to 1e30.
 // Accuracy against the platform libm is asserted in tests/test_math.py.
 #pragma once
 
@@ -202,8 +202,39 @@ MPGOS_HD MPGOS_INLINE double log(double x)
     return fma(dk, detail::LN2_HI, f - (hfsq - lo));
 }
 
-// x^y for the controller's use (x >= 0, y > 0 typical); IEEE special cases for
-// x in {0, inf, NaN}; negative x gives NaN unless y is an integer.
+namespace detail {
+// log(x) as an unevaluated sum hi + lo (absolute error ~1e-16 * |lo|-scale),
+// x > 0 finite.  ln 2 split so that k * LN2_HI_SPLIT is exact for |k| < 2^11.
+constexpr double LN2_HI_SPLIT = 0x1.62e42feep-1;
+constexpr double LN2_LO_SPLIT = 0x1.a39ef35793c76p-33;
+MPGOS_HD MPGOS_INLINE void log_dd(double x, double& hi, double& lo)
+{
+    int k = 0;
+    if (x < 0x1p-1022) { x *= 0x1p54; k = -54; }
+    const uint64_t u = bits(x);
+    k += (int)((u >> 52) & 0x7ff) - 1023;
+    double m = from_bits((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    if (m > 0x1.6a09e667f3bcdp+0) { m *= 0.5; k += 1; }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double hfsq = 0.5 * f * f;
+    const double R = log_R(s * s);
+    const double L = f - (hfsq - s * (hfsq + R));   // log(1 + f), |L| <= 0.35
+    const double dk = (double)k;
+    const double a = dk * LN2_HI_SPLIT;             // exact
+    const double sum = a + L;                       // two-sum a + L
+    const double bb = sum - a;
+    const double err = (a - (sum - bb)) + (L - bb);
+    hi = sum;
+    lo = fma(dk, LN2_LO_SPLIT, err);
+}
+} // namespace detail
+
+// x^y = exp(y log x) with log x carried in double-double and the product's
+// rounding error folded back (exp(p + e) ~ exp(p) (1 + e)), so the result is
+// within a few ulp over the controller's range (x up to 1e30, y = 1/5, 1/4).
+// IEEE special cases for x in {0, inf, NaN}; negative x gives NaN unless y is
+// an integer.
 MPGOS_HD MPGOS_INLINE double pow(double x, double y)
 {
     if (x != x || y != y) return x + y;
@@ -214,13 +245,20 @@ MPGOS_HD MPGOS_INLINE double pow(double x, double y)
         if (x > 0.0) return y > 0.0 ? x : 0.0;
         return from_bits(0x7ff8000000000000ull);
     }
+    double sign = 1.0;
     if (x < 0.0) {
         if (rint(y) != y) return from_bits(0x7ff8000000000000ull);
-        const double a = exp(y * log(-x));
         const double half = 0.5 * y;
-        return (rint(half) != half) ? -a : a;   // odd integer exponent
+        if (rint(half) != half) sign = -1.0;   // odd integer exponent
+        x = -x;
     }
-    return exp(y * log(x));
+    double lh, ll;
+    detail::log_dd(x, lh, ll);
+    const double ph = y * lh;
+    const double pl = fma(y, lh, -ph) + y * ll;
+    const double e = exp(ph);
+    if (!isfinite(e) || e == 0.0) return sign * e;
+    return sign * fma(e, pl, e);
 }
 
 // ---------------------------------------------------------------- float versions

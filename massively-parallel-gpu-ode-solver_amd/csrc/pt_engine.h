@@ -173,7 +173,9 @@ public:
         atol.assign(SD, 1e-8);
         etol.assign(NE, 1e-6);
         edir.assign(NE, 0);
-        // device buffers start from the (zeroed) host mirrors
+        // device buffers (inputs and per-instance outputs) start from the zeroed host mirrors
+        if ((rc = copy(nacc, true)) || (rc = copy(nrej, true)) || (rc = copy(status, true)) || (rc = copy(nev, true)))
+            return rc;
         return sync(MPGOS_PT_ALL, true);
     }
 

@@ -5,7 +5,8 @@
 // fused multiply-add = 2, comparisons, fabs, fmin, fmax = 0).  The library's
 // elementary functions are charged the operations their source performs
 // (csrc/mpgos_math.h): cos or sin 38 (reduction 5, both polynomial kernels
-// 19 + 14), exp 31, log 28, pow = log + 1 + exp = 60.  Used for the
+// 19 + 14), exp 31, log 28, pow 70 (double-double log 32, product 5,
+// exp 31, final fma 2).  Used for the
 // roofline's algorithmic FLOPs per step.
 #pragma once
 

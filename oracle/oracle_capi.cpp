@@ -21,7 +21,7 @@ F cos(F a) { g_count += 38; return F(mpgos::math::cos(a.v)); }
 F sin(F a) { g_count += 38; return F(mpgos::math::sin(a.v)); }
 F exp(F a) { g_count += 31; return F(mpgos::math::exp(a.v)); }
 F log(F a) { g_count += 28; return F(mpgos::math::log(a.v)); }
-F pow(F a, F b) { g_count += 60; return F(mpgos::math::pow(a.v, b.v)); }
+F pow(F a, F b) { g_count += 70; return F(mpgos::math::pow(a.v, b.v)); }
 } // namespace flops
 
 template <class Sys, class S>
@@ -41,16 +41,36 @@ static void run_all(const Job& j)
     }
 }
 
+template <class Sys>
+static void trace_one(const Job& j, int tid, typename Instance<Sys, double>::Trace* tr)
+{
+    std::vector<double> sp(Sys::NSP > 0 ? Sys::NSP : 1);
+    std::vector<int> spi(Sys::NISP > 0 ? Sys::NISP : 1);
+    for (int i = 0; i < Sys::NSP; i++) sp[i] = j.sPAR[i];
+    for (int i = 0; i < Sys::NISP; i++) spi[i] = j.sPARi[i];
+    Instance<Sys, double> inst(j, tid, sp.data(), spi.data());
+    inst.trace = tr;
+    inst.run();
+}
+
 typedef void (*Runner)(const Job&);
-struct Entry { const char* name; Runner run; Runner count; };
+struct TraceArgs { int cap; int n; double* t; double* h; int* acc; double* ratio; };
+typedef void (*Tracer)(const Job&, int, TraceArgs*);
+template <class Sys>
+static void trace_entry(const Job& j, int tid, TraceArgs* a)
+{
+    static_assert(sizeof(TraceArgs) == sizeof(typename Instance<Sys, double>::Trace), "trace layout");
+    trace_one<Sys>(j, tid, reinterpret_cast<typename Instance<Sys, double>::Trace*>(a));
+}
+struct Entry { const char* name; Runner run; Runner count; Tracer trace; };
 
 static const Entry kSystems[] = {
-    {"duffing", run_all<mpgos::systems::Duffing, double>, run_all<mpgos::systems::Duffing, flops::F>},
-    {"duffing_q", run_all<mpgos::systems::DuffingQ, double>, run_all<mpgos::systems::DuffingQ, flops::F>},
-    {"duffing_eps", run_all<mpgos::systems::DuffingEps, double>, run_all<mpgos::systems::DuffingEps, flops::F>},
-    {"duffing_dw", run_all<mpgos::systems::DuffingDW, double>, run_all<mpgos::systems::DuffingDW, flops::F>},
-    {"lorenz", run_all<mpgos::systems::Lorenz, double>, run_all<mpgos::systems::Lorenz, flops::F>},
-    {"affine2", run_all<mpgos::systems::Affine2, double>, run_all<mpgos::systems::Affine2, flops::F>},
+    {"duffing", run_all<mpgos::systems::Duffing, double>, run_all<mpgos::systems::Duffing, flops::F>, trace_entry<mpgos::systems::Duffing>},
+    {"duffing_q", run_all<mpgos::systems::DuffingQ, double>, run_all<mpgos::systems::DuffingQ, flops::F>, trace_entry<mpgos::systems::DuffingQ>},
+    {"duffing_eps", run_all<mpgos::systems::DuffingEps, double>, run_all<mpgos::systems::DuffingEps, flops::F>, trace_entry<mpgos::systems::DuffingEps>},
+    {"duffing_dw", run_all<mpgos::systems::DuffingDW, double>, run_all<mpgos::systems::DuffingDW, flops::F>, trace_entry<mpgos::systems::DuffingDW>},
+    {"lorenz", run_all<mpgos::systems::Lorenz, double>, run_all<mpgos::systems::Lorenz, flops::F>, trace_entry<mpgos::systems::Lorenz>},
+    {"affine2", run_all<mpgos::systems::Affine2, double>, run_all<mpgos::systems::Affine2, flops::F>, trace_entry<mpgos::systems::Affine2>},
 };
 
 static const Entry* find(const char* name)
@@ -89,5 +109,19 @@ int oracle_pt_count_flops(const char* system, oracle::Job* job, long long* flops
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+// Per-attempt trace of instance `tid` (time before the step, step size,
+// accepted flag, tol/err ratio of the RKCK45 controller).  Returns the number
+// of recorded attempts in *n.
+int oracle_pt_trace(const char* system, oracle::Job* job, int tid, int cap, double* t, double* h, int* acc,
+                    double* ratio, int* n)
+{
+    const oracle::Entry* e = oracle::find(system);
+    if (!e || !job || tid < 0 || tid >= job->nt) return -4;
+    oracle::TraceArgs a{cap, 0, t, h, acc, ratio};
+    e->trace(*job, tid, &a);
+    *n = a.n;
+    return 0;
+}
 
 } // extern "C"

@@ -73,6 +73,10 @@ struct Instance {
     int doIdx, doSkipped, stop, userStop;
     long long accepted = 0, rejected = 0;
     int flags = 0, events = 0;
+    // optional per-attempt trace (tests): time, step, accepted, tol/err ratio
+    struct Trace { int cap; int n; double* t; double* h; int* acc; double* ratio; };
+    Trace* trace = nullptr;
+    S lastRatio = S(0.0);
 
     Instance(const Job& j, int id, S* sp, int* spi) : J(j), tid(id), sPAR(sp), sPARi(spi) {}
 
@@ -218,6 +222,7 @@ struct Instance {
             accept = accept && (err[i] < tol);
             ratio = fmn(ratio, tol / err[i]);
         }
+        lastRatio = ratio;
         S factor = S(0.9) * pw(ratio, accept == 1 ? 1.0 / 5.0 : 1.0 / 4.0);
         if (!fin(factor)) finite = false;
         const bool tiny = dt < S(J.dtmin) * S(1.01);
@@ -292,6 +297,13 @@ struct Instance {
                 }
             }
 
+            if (trace && trace->n < trace->cap) {
+                trace->t[trace->n] = value(t);
+                trace->h[trace->n] = value(dt);
+                trace->acc[trace->n] = accept;
+                trace->ratio[trace->n] = value(lastRatio);
+                trace->n++;
+            }
             if (accept != 1) { rejected++; continue; }
 
             t = t + dt;

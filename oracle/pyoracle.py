@@ -61,7 +61,7 @@ def _ptr(a, typ):
     return a.ctypes.data_as(typ) if a is not None and a.size else ctypes.cast(None, typ)
 
 
-def run(system, spec, nthreads=0, count_flops=False):
+def run(system, spec, nthreads=0, count_flops=False, _job_hook=None):
     """Integrate with the oracle.  `spec` keys: algorithm, nt, ndo, n_active,
     options (dict), rtol, atol, etol, edir, evcap, and the SoA arrays TD, X, T,
     cPAR, sPAR, sPARi, ACC, ACCi, DOidx, DOt, DOx (float64 / int32).  Returns a
@@ -110,6 +110,9 @@ def run(system, spec, nthreads=0, count_flops=False):
         setattr(j, k, _ptr(out[k], _ip))
     j.nacc, j.nrej = _ptr(out["nacc"], _lp), _ptr(out["nrej"], _lp)
     j.nthreads = int(nthreads)
+    if _job_hook is not None:
+        _job_hook(j)
+        return out
     if count_flops:
         fl = ctypes.c_longlong()
         rc = lib().oracle_pt_count_flops(system.encode(), ctypes.byref(j), ctypes.byref(fl))
@@ -125,3 +128,32 @@ def run(system, spec, nthreads=0, count_flops=False):
 
 def max_threads():
     return lib().oracle_max_threads()
+
+
+def trace(system, spec, tid=0, cap=100000):
+    """Per-attempt trace of one instance: arrays t (before the attempt), h,
+    accepted flag and the controller's tol/err ratio."""
+    L = lib()
+    if not hasattr(L, "_trace_set"):
+        L.oracle_pt_trace.restype = ctypes.c_int
+        L.oracle_pt_trace.argtypes = [ctypes.c_char_p, ctypes.POINTER(Job), ctypes.c_int, ctypes.c_int,
+                                      _dp, _dp, _ip, _dp, ctypes.POINTER(ctypes.c_int)]
+        L._trace_set = True
+    t = np.zeros(cap)
+    h = np.zeros(cap)
+    acc = np.zeros(cap, np.int32)
+    ratio = np.zeros(cap)
+    n = ctypes.c_int()
+    # reuse run()'s marshalling through a one-instance spec
+    holder = {}
+
+    def hook(j):
+        rc = L.oracle_pt_trace(system.encode(), ctypes.byref(j), int(tid), int(cap), _ptr(t, _dp), _ptr(h, _dp),
+                               _ptr(acc, _ip), _ptr(ratio, _dp), ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError("oracle trace failed")
+        holder["n"] = n.value
+
+    run(system, spec, _job_hook=hook)
+    k = holder["n"]
+    return {"t": t[:k], "h": h[:k], "accepted": acc[:k], "ratio": ratio[:k]}

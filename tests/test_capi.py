@@ -1,0 +1,74 @@
+"""The C ABI without a GPU: libmpgos.so loads, exports every entry point that
+include/mpgos.h declares, reports errors through return codes instead of
+exiting, and the enumerations match the reference's order."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import mpgos
+from mpgos import ListOfSolverOptions, ListOfVariables
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", f)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(mpgos_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", mpgos.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (mpgos_\w+)", out))
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+    L = mpgos.lib()
+    for s in syms:
+        getattr(L, s)
+
+
+def test_registered_systems():
+    assert set(mpgos.list_systems()) >= {"duffing", "duffing_q", "duffing_eps", "duffing_dw", "lorenz", "affine2"}
+
+
+def test_errors_are_return_codes():
+    L = mpgos.lib()
+    h = ctypes.c_void_p()
+    rc = L.mpgos_pt_create(b"no_such_system", 1, 0, 16, 0, 0, ctypes.byref(h))
+    assert rc == -4 and b"unknown system" in L.mpgos_last_error()
+    assert L.mpgos_pt_solve(None) == -1
+    assert L.mpgos_pt_destroy(None) == -1
+    assert L.mpgos_math_eval_host(9, None, None, None, 0) == -1
+
+
+def test_enumerations_follow_the_reference_order():
+    # SingleSystem_PerThread_Interface.cuh:35-62
+    ref_vars = ["All", "TimeDomain", "ActualState", "ActualTime", "ControlParameters", "SharedParameters",
+                "IntegerSharedParameters", "Accessories", "IntegerAccessories", "DenseOutput", "DenseIndex",
+                "DenseTime", "DenseState"]
+    ref_opts = ["ThreadsPerBlock", "InitialTimeStep", "ActiveNumberOfThreads", "MaximumTimeStep",
+                "MinimumTimeStep", "TimeStepGrowLimit", "TimeStepShrinkLimit", "RelativeTolerance",
+                "AbsoluteTolerance", "EventTolerance", "EventDirection", "DenseOutputMinimumTimeStep",
+                "DenseOutputSaveFrequency", "PreferSharedMemory"]
+    assert [ListOfVariables(i).name for i in range(len(ref_vars))] == ref_vars
+    assert [ListOfSolverOptions(i).name for i in range(len(ref_opts))] == ref_opts
+    header = open(os.path.join(ROOT, "include", "mpgos.h")).read()
+    assert re.search(r"MPGOS_PT_ALL = 0,\s*MPGOS_PT_TIME_DOMAIN,\s*MPGOS_PT_ACTUAL_STATE,", header)
+
+
+def test_create_without_device_fails_cleanly():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    L = mpgos.lib()
+    h = ctypes.c_void_p()
+    rc = L.mpgos_pt_create(b"lorenz", 1, 0, 16, 0, 0, ctypes.byref(h))
+    assert rc == -2 and not h.value

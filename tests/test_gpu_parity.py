@@ -43,3 +43,107 @@ def test_config2_events_subset(gpu, oracle):
     out = run_gpu(spec)
     assert_bitwise(out, ref, keys=("X", "T", "nacc", "nrej", "status", "nev", "evlog", "evlogidx"), label="C2")
     assert ref["nev"].sum() > 0
+
+
+# ---------------------------------------------------------------- scenario sweep
+import scenarios as S  # noqa: E402
+
+_ALL = S.all_scenarios()
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["plain", "persistent"])
+@pytest.mark.parametrize("name,system,spec", _ALL, ids=[x[0] for x in _ALL])
+def test_scenarios_bitwise(gpu, oracle, name, system, spec, mode):
+    ref = oracle.run(system, spec)
+    out = run_gpu(spec, kernel_mode=mode)
+    keys = ("X", "T", "TD", "cPAR", "ACC", "ACCi", "nacc", "nrej", "status", "nev", "DOidx", "DOt", "DOx",
+            "evlog", "evlogidx")
+    assert_bitwise(out, ref, keys=keys, label=f"{name}/{mode}")
+
+
+@pytest.mark.parametrize("block,variant", [(64, 1), (128, 2), (256, 3), (32, 0), (96, 0)])
+def test_launch_shapes_do_not_change_results(gpu, oracle, block, variant):
+    spec = W.config1_lorenz(nt=3000)
+    ref = oracle.run("lorenz", spec)
+    for mode in (0, 1):
+        out = run_gpu(spec, kernel_mode=mode, block=block, variant=variant)
+        assert_bitwise(out, ref, label=f"lorenz block={block} variant={variant} mode={mode}")
+
+
+def test_persistent_with_fewer_lanes_than_instances(gpu, oracle):
+    # one wave per SIMD: lanes are refilled from the queue many times
+    spec = W.config1_lorenz(nt=70000)
+    ref = oracle.run("lorenz", spec)
+    out = run_gpu(spec, kernel_mode=1, waves_per_simd=1)
+    assert_bitwise(out, ref, label="lorenz persistent refill")
+
+
+def test_graph_replays_match_eager(gpu, oracle):
+    from mpgos import ListOfSolverOptions as O, ListOfVariables as V, ProblemSolver
+    spec = W.config1_lorenz(nt=2048)
+    ref = oracle.run("lorenz", spec)
+    for mode in (0, 1):
+        s = ProblemSolver("lorenz", spec["nt"], Algorithms.RKCK45)
+        try:
+            W.load(s, spec)
+            s.SolverOption(O.KernelMode, mode)
+            s.SolverOption(O.GraphCapture, 1)
+            for rep in range(3):
+                W.load(s, spec)                      # eager uploads between replays
+                s.solve_blocking()
+                out = W.read(s)
+                assert_bitwise(out, ref, label=f"graph replay {rep} mode {mode}")
+        finally:
+            s.close()
+
+
+def test_repeated_solves_continue_the_integration(gpu, oracle):
+    # the reference tutorials chain Solve() calls on the same object (Reference.cu:100-117)
+    spec = W.config0_duffing_rk4(nt=256)
+    spec["TD"][1] = 2 * W.PI
+    outs = run_gpu(spec, solves=3)
+    cur = dict(spec)
+    for k, out in enumerate(outs):
+        ref = oracle.run("duffing", cur)
+        assert_bitwise(out, ref, label=f"solve {k}")
+        cur = dict(cur)
+        cur["X"], cur["T"], cur["ACC"] = ref["X"], ref["T"], ref["ACC"]
+
+
+def test_device_math_matches_host_bitwise(gpu):
+    import ctypes
+    import torch
+    from test_math import FN, host_eval
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-3e4, 3e4, 100000), rng.uniform(-5, 5, 100000), 10.0 ** rng.uniform(-300, 300, 1000)])
+    y = rng.uniform(0.1, 0.3, x.size)
+    L = gpu.lib()
+    for name, fn in FN.items():
+        arg = np.abs(x) if name in ("log", "pow") else (np.clip(x, -700, 700) if name == "exp" else x)
+        xd = torch.from_numpy(arg).cuda()
+        yd = torch.from_numpy(y).cuda()
+        od = torch.empty_like(xd)
+        assert L.mpgos_math_eval_device(fn, ctypes.c_void_p(xd.data_ptr()), ctypes.c_void_p(yd.data_ptr()),
+                                        ctypes.c_void_p(od.data_ptr()), arg.size) == 0
+        dev = od.cpu().numpy()
+        host = host_eval(name, arg, y if name == "pow" else None)
+        assert np.array_equal(dev.view(np.uint64), host.view(np.uint64)), name
+
+
+@pytest.mark.parametrize("precision", [1, 2], ids=["float", "float_kahan"])
+def test_fp32_modes_error_against_fp64_oracle(gpu, oracle, precision):
+    # FP32 stepping: error against the FP64 oracle is reported, not zero
+    spec = W.config0_duffing_rk4(nt=512)
+    spec["TD"][1] = 4 * (2 * W.PI)
+    ref = oracle.run("duffing", spec)
+    out = run_gpu(spec, precision=precision)
+    err = np.abs(out["X"].astype(np.float64) - ref["X"]).max()
+    assert (out["nacc"] == ref["nacc"]).all()
+    assert err < (5e-3 if precision == 1 else 5e-4), err
+    spec = W.config1_lorenz(nt=512)
+    spec["TD"][1] = 1.0
+    spec["rtol"] = spec["atol"] = 1e-6
+    ref = oracle.run("lorenz", spec)
+    out = run_gpu(spec, precision=precision)
+    rel = np.abs(out["X"].astype(np.float64) - ref["X"]).max() / np.abs(ref["X"]).max()
+    assert rel < 1e-3, rel

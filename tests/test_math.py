@@ -1,0 +1,110 @@
+"""The library's elementary functions (csrc/mpgos_math.h, host build inside
+libmpgos.so) against the platform libm / numpy and high-precision mpmath.
+They replace the CUDA libm calls of the reference (pow in
+SingleSystem_PerThread_ExplicitRungeKutta_ErrorControllers.cuh:51-53, cos in
+the user right-hand sides), whose exact bits are not pinned by anything in
+the reference; what is pinned here is accuracy: a few ulp everywhere the
+solver uses them."""
+import ctypes
+
+import mpmath
+import numpy as np
+import pytest
+
+from mpgos import lib
+
+FN = {"cos": 0, "sin": 1, "exp": 2, "log": 3, "pow": 4}
+
+
+def host_eval(name, x, y=None):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    yp = None
+    if y is not None:
+        y = np.ascontiguousarray(np.broadcast_to(y, x.shape), np.float64)
+        yp = y.ctypes.data
+    rc = lib().mpgos_math_eval_host(FN[name], x.ctypes.data, yp, out.ctypes.data, x.size)
+    assert rc == 0
+    return out
+
+
+def ulp_err(a, ref):
+    ref = np.asarray(ref, np.float64)
+    return np.abs(a - ref) / np.spacing(np.abs(ref))
+
+
+def mp_ref(fn, xs, ys=None):
+    mpmath.mp.dps = 40
+    if ys is None:
+        return np.array([float(fn(mpmath.mpf(float(v)))) for v in xs])
+    return np.array([float(fn(mpmath.mpf(float(a)), mpmath.mpf(float(b)))) for a, b in zip(xs, ys)])
+
+
+rng = np.random.default_rng(1)
+
+
+@pytest.mark.parametrize("name,mfn", [("cos", mpmath.cos), ("sin", mpmath.sin)])
+def test_trig_against_mpmath(name, mfn):
+    # forcing arguments reach ~2.6e4 (uneven-time-scale workload); stress up to 1e5
+    x = np.concatenate([rng.uniform(-4, 4, 400), rng.uniform(-3e4, 3e4, 400), rng.uniform(-1e5, 1e5, 200),
+                        10.0 ** rng.uniform(-12, -1, 100)])
+    got = host_eval(name, x)
+    ref = mp_ref(mfn, x)
+    big = np.abs(ref) > 1e-3
+    assert ulp_err(got[big], ref[big]).max() <= 2.0
+    # near zeros of the function the absolute error is what counts
+    assert np.abs(got - ref).max() <= 4e-16 * max(1.0, 1e-5 * np.abs(x).max())
+
+
+def test_trig_bulk_against_numpy():
+    x = np.concatenate([np.linspace(-500, 500, 200001), rng.uniform(-3e4, 3e4, 200000)])
+    for name, f in (("cos", np.cos), ("sin", np.sin)):
+        got = host_eval(name, x)
+        ref = f(x)
+        assert np.abs(got - ref).max() < 5e-16 * 4, name
+
+
+def test_trig_identities_and_special_values():
+    x = rng.uniform(-1e3, 1e3, 10000)
+    c, s = host_eval("cos", x), host_eval("sin", x)
+    assert np.abs(c * c + s * s - 1).max() < 4e-16
+    assert np.isnan(host_eval("cos", np.array([np.inf, -np.inf, np.nan]))).all()
+    assert host_eval("cos", np.array([0.0]))[0] == 1.0
+    assert host_eval("sin", np.array([-0.0, 1e-300]))[1] == 1e-300
+
+
+def test_exp_log():
+    x = np.concatenate([rng.uniform(-700, 700, 2000), rng.uniform(-1, 1, 1000), [0.0, 1e-300, -745.0, 709.7]])
+    assert ulp_err(host_eval("exp", x), mp_ref(mpmath.exp, x)).max() <= 2.0
+    y = np.concatenate([10.0 ** rng.uniform(-300, 300, 2000), rng.uniform(0.5, 2.0, 1000),
+                        [5e-324, 2.2250738585072014e-308, 1.0 + 2**-52, 1e308]])
+    ly = host_eval("log", y)
+    ref = mp_ref(mpmath.log, y)
+    ok = np.abs(ref) > 1e-3
+    assert ulp_err(ly[ok], ref[ok]).max() <= 2.0
+    assert np.abs(ly - ref).max() < 1e-15 * np.abs(ref).max()
+    assert host_eval("log", np.array([1.0]))[0] == 0.0
+    assert np.isnan(host_eval("log", np.array([-1.0, np.nan]))).all()
+    assert host_eval("log", np.array([0.0]))[0] == -np.inf
+    assert host_eval("exp", np.array([1000.0]))[0] == np.inf
+    assert host_eval("exp", np.array([-1000.0]))[0] == 0.0
+
+
+def test_pow_controller_range():
+    # the controller evaluates R^(1/5) and R^(1/4) for R = min(tol/err) in [0, 1e30]
+    r = np.concatenate([10.0 ** rng.uniform(-20, 30, 3000), [1.0, 1e30, 0.9 ** -5]])
+    for e in (0.2, 0.25):
+        got = host_eval("pow", r, e)
+        ref = mp_ref(mpmath.power, r, np.full_like(r, e))
+        assert ulp_err(got, ref).max() <= 4.0, e
+    assert host_eval("pow", np.array([0.0]), 0.2)[0] == 0.0
+    assert host_eval("pow", np.array([np.inf]), 0.2)[0] == np.inf
+    assert np.isnan(host_eval("pow", np.array([np.nan]), 0.2)[0])
+    assert host_eval("pow", np.array([-2.0]), 3.0)[0] == -8.0
+
+
+def test_float_versions_round_once():
+    # FP32 mode: float functions are the double ones rounded once
+    x = rng.uniform(-100, 100, 1000).astype(np.float32).astype(np.float64)
+    assert np.array_equal(host_eval("cos", x).astype(np.float32), np.cos(x).astype(np.float32)) or \
+        np.abs(host_eval("cos", x).astype(np.float32) - np.cos(x).astype(np.float32)).max() <= 6e-8
