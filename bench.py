@@ -14,7 +14,8 @@ Printed (rank 0, one JSON line): the contract fields plus
                    solver's stream) against the 78.6 TFLOP/s FP64 peak;
   cpu_baseline  -- the FP64 OpenMP oracle on the host cores, same workload;
   parity        -- the GPU result compared bit for bit with that oracle run;
-  forms         -- plain (reference form) and persistent work-queue kernels.
+  forms         -- the plain kernel in the reference's launch shape and the
+                   configuration chosen by the autotuner (mpgos/tuning.py).
 """
 import argparse
 import ctypes
@@ -32,76 +33,48 @@ FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 
-def _hip():
-    h = ctypes.CDLL("libamdhip64.so")
-    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-    h.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
-    h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
-    h.hipFree.argtypes = [ctypes.c_void_p]
-    return h
-
-
-class Resettable:
-    """A solver plus device copies of its initial state, so each step starts
-    from identical inputs without touching the host."""
-
-    def __init__(self, mpgos, spec, mode, variant=0, block=None, graph=0):
-        from mpgos import ListOfSolverOptions as O, ListOfVariables as V, workloads as W
-        self.V = V
-        self.s = mpgos.ProblemSolver(spec["system"], spec["nt"], spec["algorithm"])
-        W.load(self.s, spec)
-        self.s.SolverOption(O.KernelMode, mode)
-        self.s.SolverOption(O.LaunchVariant, variant)
-        self.s.SolverOption(O.GraphCapture, graph)
-        if block:
-            self.s.SolverOption(O.ThreadsPerBlock, block)
-        self.hip = _hip()
-        self.saved = []
-        for var in (V.ActualState, V.ActualTime, V.TimeDomain):
-            ptr, n = self.s.device_ptr(var)
-            dst = ctypes.c_void_p()
-            assert self.hip.hipMalloc(ctypes.byref(dst), n) == 0
-            assert self.hip.hipMemcpyAsync(dst, ptr, n, 3, self.s.stream()) == 0
-            self.saved.append((ptr, dst, n))
-        self.sync()
-
-    def sync(self):
-        assert self.hip.hipStreamSynchronize(self.s.stream()) == 0
-
-    def step(self):
-        for ptr, src, n in self.saved:
-            assert self.hip.hipMemcpyAsync(ptr, src, n, 3, self.s.stream()) == 0
-        self.s.Solve()
-
-    def close(self):
-        self.sync()
-        for _, src, _ in self.saved:
-            self.hip.hipFree(src)
-        self.s.close()
-
-
-def time_form(mpgos, spec, mode, steps, warmup, variant=0, block=None, graph=0):
-    r = Resettable(mpgos, spec, mode, variant, block, graph)
+def time_form(mpgos, spec, cfg, steps, warmup):
+    """K timed steps (device-side input reset + Solve) of one launch configuration."""
+    from mpgos import workloads as W
+    from mpgos.tuning import DeviceSnapshot, apply
+    s = mpgos.ProblemSolver(spec["system"], spec["nt"], spec["algorithm"])
     try:
+        W.load(s, spec)
+        apply(s, cfg)
+        snap = DeviceSnapshot(s)
         for _ in range(warmup):
-            r.step()
-        r.sync()
+            snap.restore()
+            s.Solve()
+        snap.sync()
         kernel_ms = []
         t0 = time.perf_counter()
         for _ in range(steps):
-            r.step()
-            r.sync()
-            kernel_ms.append(r.s.last_solve_ms())
+            snap.restore()
+            s.Solve()
+            snap.sync()
+            kernel_ms.append(s.last_solve_ms())
         t1 = time.perf_counter()
-        r.s.SynchroniseFromDeviceToHost(r.V.All)
-        r.s.InsertSynchronisationPoint()
-        r.s.SynchroniseSolver()
-        from mpgos import workloads as W
-        out = W.read(r.s)
+        snap.close()
+        s.SynchroniseFromDeviceToHost(mpgos.ListOfVariables.All)
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
+        out = W.read(s)
     finally:
-        r.close()
+        s.close()
     return {"wall_ms_per_step": (t1 - t0) * 1e3 / steps, "kernel_ms": float(np.mean(kernel_ms)),
-            "kernel_ms_min": float(np.min(kernel_ms)), "out": out}
+            "kernel_ms_min": float(np.min(kernel_ms)), "out": out, "config": cfg}
+
+
+def tune(mpgos, spec):
+    from mpgos import workloads as W
+    from mpgos.tuning import autotune
+    s = mpgos.ProblemSolver(spec["system"], spec["nt"], spec["algorithm"])
+    try:
+        W.load(s, spec)
+        best, table = autotune(s, log=lambda m: print(m, file=sys.stderr))
+    finally:
+        s.close()
+    return best, table
 
 
 def main():
@@ -111,7 +84,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nt", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--forms", default="plain,persistent")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the solver path has no inter-GPU work "
@@ -123,10 +96,12 @@ def main():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
     spec = W.config1_lorenz(nt=args.nt)
-    forms = {}
-    for name in args.forms.split(","):
-        mode = {"plain": 0, "persistent": 1}[name]
-        forms[name] = time_form(mpgos, spec, mode, args.steps, args.warmup)
+    forms = {"plain": time_form(mpgos, spec, {"mode": 0, "block": 64}, args.steps, args.warmup)}
+    if not args.no_tune:
+        best_cfg, table = tune(mpgos, spec)
+        forms["autotuned"] = time_form(mpgos, spec, best_cfg, args.steps, args.warmup)
+    else:
+        forms["persistent"] = time_form(mpgos, spec, {"mode": 1, "block": 256, "round": 32}, args.steps, args.warmup)
     best = min(forms, key=lambda k: forms[k]["wall_ms_per_step"])
     res = forms[best]
     out = res["out"]
@@ -176,14 +151,14 @@ def main():
         "data": "synthetic (seeded SplitMix64, seed 20261202, stream 1)",
         "config": {"workload": f"lorenz_rkck45_{args.nt}", "system": "lorenz", "algorithm": "RKCK45",
                    "instances": args.nt, "rtol": 1e-10, "atol": 1e-10, "t_end": 10.0,
-                   "kernel_form": best, "accepted_steps_per_solve": accepted,
+                   "kernel_form": best, "launch": res["config"], "accepted_steps_per_solve": accepted,
                    "rejected_steps_per_solve": attempted - accepted},
         "roofline": {"bound": "fp64-valu", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "flops_per_attempted_step": flops_per_attempt, "kernel_ms": res["kernel_ms"]},
         "cpu_baseline": cpu,
         "parity": parity,
-        "forms": {k: {"wall_ms_per_step": v["wall_ms_per_step"], "kernel_ms": v["kernel_ms"],
+        "forms": {k: {"config": v["config"], "wall_ms_per_step": v["wall_ms_per_step"], "kernel_ms": v["kernel_ms"],
                       "instance_steps_per_s": accepted / (v["wall_ms_per_step"] * 1e-3)} for k, v in forms.items()},
     }
     if cpu:

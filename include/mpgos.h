@@ -95,11 +95,12 @@ enum {
     MPGOS_PT_DENSE_OUTPUT_SAVE_FREQUENCY,
     MPGOS_PT_PREFER_SHARED_MEMORY,
     /* new */
-    MPGOS_PT_KERNEL_MODE,          /* 0 = one lane per instance (reference form), 1 = persistent work queue */
+    MPGOS_PT_KERNEL_MODE,          /* 0 = one lane per instance (reference form), 1 = persistent work queue with lane compaction */
     MPGOS_PT_PERSISTENT_WAVES_PER_SIMD, /* persistent grid size in waves per SIMD (1..8), default from occupancy */
     MPGOS_PT_GRAPH_CAPTURE,        /* 1 = Solve() replays a captured hipGraph of its launches */
     MPGOS_PT_EVENT_LOG_CAPACITY,   /* hits logged per instance (allocates EVENT_LOG), default 0 */
     MPGOS_PT_LAUNCH_VARIANT,       /* register-budget variant of the kernel (0 = default, 1..3 = waves-per-EU 2,4,8) */
+    MPGOS_PT_PERSISTENT_ROUND_STEPS, /* persistent form: attempted steps between lane compactions (default 32) */
     MPGOS_PT_NUM_OPTIONS
 };
 

@@ -39,7 +39,8 @@
 namespace mpgos {
 namespace pt {
 
-constexpr int kMaxBlock = 256;
+constexpr int kMaxBlock = 512;
+constexpr int kMaxWaves = kMaxBlock / 64;
 
 template <int N> struct Pos { static constexpr int value = N > 0 ? N : 1; };
 
@@ -56,6 +57,7 @@ struct Args {
     unsigned int* queue;      // persistent form: [0] = next instance
     int NT, nActive, NDO, evcap, preferShared;
     int doFreq;
+    int roundSteps;           // persistent form: attempted steps between compactions
     CP dt0, dtmax, dtmin, grow, shrink;
     P doMinDt;
     CP rtol[Pos<SD>::value], atol[Pos<SD>::value];
@@ -445,51 +447,112 @@ __device__ __forceinline__ void plain_body(const Args<Sys, P, typename Lane<Sys,
     }
 }
 
-// Persistent form: each lane repeatedly takes the next instance from the
-// device queue.  A wavefront ballot finds lanes whose instance terminated;
-// one atomic per wavefront reserves as many new instances as there are idle
-// lanes and the lanes take consecutive indices by their rank in the ballot.
-// Per-instance arithmetic is independent of the lane it runs on, so results
-// are bit-identical to the plain form.  Every lane exits once the queue is
-// drained and its last instance has finished.
+// Persistent form with lane compaction.  Lanes pull instances from a device
+// work queue; every `roundSteps` attempted steps the workgroup
+//   1. counts its live lanes (wavefront ballots, prefix over waves in LDS),
+//   2. reserves as many fresh instances as it has idle lanes (one atomic per
+//      workgroup),
+//   3. packs the live instances into the lowest lanes through LDS, followed by
+//      the fresh ones, so the waves above the packed range have nothing to do
+//      and sleep at the barrier instead of issuing instructions for idle lanes.
+// Every instance's arithmetic is independent of the lane that runs it, so the
+// results are bit-identical to the plain form.  The workgroup leaves the loop
+// together once the queue is drained and no lane is live (block-uniform test).
 template <class Sys, class P, bool KAHAN, int ALG>
 __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane<Sys, P, KAHAN, ALG>::CP>& a)
 {
+    using L_t = Lane<Sys, P, KAHAN, ALG>;
+    constexpr int NW = (int)(sizeof(L_t) / 4);       // lane state in 32-bit words
+    constexpr int CW = 4;                             // words moved per LDS pass
+    constexpr int NCH = (NW + CW - 1) / CW;
     P* sPAR; int* sPARi;
     stage_shared(a, sPAR, sPARi);
-    Lane<Sys, P, KAHAN, ALG> L;
-    bool busy = false;
-    bool drained = false;          // wave-uniform
+
+    extern __shared__ __align__(16) unsigned char mpgos_dyn_lds[];
+    const size_t sharedBytes = ((size_t)Sys::NSP * sizeof(P) + (size_t)Sys::NISP * sizeof(int) + 15) & ~(size_t)15;
+    unsigned* ctl = reinterpret_cast<unsigned*>(mpgos_dyn_lds + sharedBytes);   // [0..15] wave counts, [16] base, [17] fresh, [18] drained
+    unsigned* xch = ctl + 32;                                                  // CW * blockDim.x words
+
+    const int B = blockDim.x;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int nwaves = (B + 63) >> 6;
     const unsigned lane = __lane_id();
+
+    L_t L;
+    bool busy = false;
+    if (tid == 0) ctl[18] = 0u;
     for (;;) {
-        if (!drained) {
-            const unsigned long long idle = __ballot(!busy);
-            if (idle != 0ull) {
-                const unsigned nidle = (unsigned)__popcll(idle);
-                unsigned base = 0;
-                if (lane == (unsigned)(__ffsll((long long)idle) - 1))
-                    base = atomicAdd(a.queue, nidle);
-                base = __shfl(base, __ffsll((long long)idle) - 1);
-                if (base + nidle >= (unsigned)a.nActive) drained = true;
-                if (!busy) {
-                    const unsigned rank = (unsigned)__popcll(idle & ((1ull << lane) - 1ull));
-                    const unsigned id = base + rank;
-                    if (id < (unsigned)a.nActive) {
-                        L.load(a, (int)id, sPAR, sPARi);
-                        busy = true;
-                    }
+        // ---- 1. count live lanes
+        const unsigned long long live_mask = __ballot(busy);
+        if (lane == 0) ctl[wave] = (unsigned)__popcll(live_mask);
+        __syncthreads();
+        unsigned before = 0, live = 0;
+        for (int w = 0; w < nwaves; w++) {
+            const unsigned c = ctl[w];
+            if (w < wave) before += c;
+            live += c;
+        }
+        const unsigned rank = before + (unsigned)__popcll(live_mask & ((1ull << lane) - 1ull));
+        // ---- 2. reserve fresh instances for the idle lanes
+        if (tid == 0) {
+            unsigned base = 0, fresh = 0;
+            const unsigned need = (unsigned)B - live;
+            if (!ctl[18] && need > 0) {
+                base = atomicAdd(a.queue, need);
+                if (base < (unsigned)a.nActive) fresh = min(need, (unsigned)a.nActive - base);
+                if (base + need >= (unsigned)a.nActive) ctl[18] = 1u;
+            }
+            ctl[16] = base;
+            ctl[17] = fresh;
+        }
+        __syncthreads();
+        const unsigned base = ctl[16], fresh = ctl[17];
+        if (live == 0 && fresh == 0) break;           // block-uniform: queue drained, nothing live
+        // ---- 3. pack live states into lanes [0, live), fresh ones into [live, live + fresh)
+        uint32_t w[NCH * CW];
+        __builtin_memcpy(w, &L, sizeof(L_t));
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+            if (busy) {
+#pragma unroll
+                for (int j = 0; j < CW; j++) xch[j * B + rank] = w[c * CW + j];
+            }
+            __syncthreads();
+            if ((unsigned)tid < live) {
+#pragma unroll
+                for (int j = 0; j < CW; j++) w[c * CW + j] = xch[j * B + tid];
+            }
+            __syncthreads();
+        }
+        if ((unsigned)tid < live) {
+            __builtin_memcpy(&L, w, sizeof(L_t));
+            busy = true;
+        } else if ((unsigned)tid < live + fresh) {
+            L.load(a, (int)(base + ((unsigned)tid - live)), sPAR, sPARi);
+            busy = true;
+        } else {
+            busy = false;
+        }
+        // ---- 4. integrate
+        for (int m = 0; m < a.roundSteps; m++) {
+            if (!__any(busy)) break;
+            if (busy) {
+                L.step(a, sPAR, sPARi);
+                if (L.term) {
+                    L.finish(a, sPAR, sPARi);
+                    busy = false;
                 }
             }
         }
-        if (!__any(busy)) break;
-        if (busy) {
-            L.step(a, sPAR, sPARi);
-            if (L.term) {
-                L.finish(a, sPAR, sPARi);
-                busy = false;
-            }
-        }
     }
+}
+
+// LDS bytes of the persistent form beyond the shared parameters.
+template <class Sys, class P, bool KAHAN, int ALG>
+constexpr size_t persistent_lds_bytes(int block)
+{
+    return 32 * 4 + (size_t)4 * 4 * block;
 }
 
 // Resets the work-queue head on the stream (a kernel, so the sequence can be

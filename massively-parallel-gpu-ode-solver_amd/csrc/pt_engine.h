@@ -112,11 +112,14 @@ public:
 
     // options (SingleSystem_PerThread_Interface.cuh:340-345, 475-496 defaults)
     int blockSize = 64, nActive = 0, preferShared = 1;
+    bool blockSet = false;                 // ThreadsPerBlock given explicitly
+    static constexpr int kPersistentBlock = 256;
+    int block() const { return (kernelMode == MPGOS_KERNEL_PLAIN || blockSet) ? blockSize : kPersistentBlock; }
     double dt0 = 1e-2, dtmax = 1.0e6, dtmin = 1.0e-12, grow = 5.0, shrink = 0.1, doMinDt = 0.0;
     int doFreq = 1;
     std::vector<double> rtol, atol, etol;
     std::vector<int> edir;
-    int kernelMode = MPGOS_KERNEL_PLAIN, wavesPerSimd = 0, graphCapture = 0, evcap = 0, variant = 0;
+    int kernelMode = MPGOS_KERNEL_PLAIN, wavesPerSimd = 0, graphCapture = 0, evcap = 0, variant = 0, roundSteps = 32;
 
     Buffer<P> TD, X, T, cPAR, sPAR, ACC, DOt, DOx, evlog;
     Buffer<int> sPARi, ACCi, DOidx, status, nev, evlogidx;
@@ -204,8 +207,8 @@ public:
     {
         switch (opt) {
         case MPGOS_PT_THREADS_PER_BLOCK:
-            if ((int)v < 1 || (int)v > kMaxBlock) { set_error("ThreadsPerBlock must be in 1..256"); return MPGOS_E_ARG; }
-            blockSize = (int)v; return MPGOS_OK;
+            if ((int)v < 1 || (int)v > kMaxBlock) { set_error("ThreadsPerBlock must be in 1..512"); return MPGOS_E_ARG; }
+            blockSize = (int)v; blockSet = true; return MPGOS_OK;
         case MPGOS_PT_INITIAL_TIME_STEP: dt0 = v; return MPGOS_OK;
         case MPGOS_PT_ACTIVE_NUMBER_OF_THREADS:
             if ((int)v < 0 || (int)v > NT) { set_error("ActiveNumberOfThreads must be in 0..NT"); return MPGOS_E_ARG; }
@@ -236,6 +239,9 @@ public:
             if ((int)v < 0 || (int)v > 8) { set_error("PersistentWavesPerSimd must be in 0..8"); return MPGOS_E_ARG; }
             wavesPerSimd = (int)v; return MPGOS_OK;
         case MPGOS_PT_GRAPH_CAPTURE: graphCapture = (int)v != 0; return MPGOS_OK;
+        case MPGOS_PT_PERSISTENT_ROUND_STEPS:
+            if ((int)v < 1) { set_error("PersistentRoundSteps must be >= 1"); return MPGOS_E_ARG; }
+            roundSteps = (int)v; return MPGOS_OK;
         case MPGOS_PT_LAUNCH_VARIANT:
             if ((int)v < 0 || (int)v > 3) { set_error("LaunchVariant must be in 0..3"); return MPGOS_E_ARG; }
             variant = (int)v; return MPGOS_OK;
@@ -436,6 +442,7 @@ public:
         a.evlog = evlog.d; a.evlogidx = evlogidx.d; a.queue = queue;
         a.NT = NT; a.nActive = nActive; a.NDO = NDO; a.evcap = evcap; a.preferShared = preferShared;
         a.doFreq = doFreq;
+        a.roundSteps = roundSteps;
         a.dt0 = (CP)(P)dt0;   // options are stored as Precision in the reference (PT-IF:619-650)
         a.dtmax = (CP)(P)dtmax; a.dtmin = (CP)(P)dtmin; a.grow = (CP)(P)grow; a.shrink = (CP)(P)shrink;
         if (KAHAN) { a.dt0 = (CP)dt0; a.dtmax = (CP)dtmax; a.dtmin = (CP)dtmin; a.grow = (CP)grow; a.shrink = (CP)shrink; }
@@ -470,24 +477,32 @@ public:
                                                 : pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PERSISTENT>(variant);
     }
 
-    size_t dyn_lds() const { return preferShared ? (size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int) : 0; }
+    size_t dyn_lds() const
+    {
+        const size_t sharedBytes = preferShared ? (size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int) : 0;
+        if (kernelMode == MPGOS_KERNEL_PLAIN) return sharedBytes;
+        // persistent form: shared parameters (16-byte aligned), control words, exchange buffer
+        return ((((size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int)) + 15) & ~(size_t)15) +
+               persistent_lds_bytes<Sys, P, KAHAN, MPGOS_RK4>(block());
+    }
 
     int launch_geometry(KernelFn fn, int& grid) const
     {
+        const int B = block();
         if (kernelMode == MPGOS_KERNEL_PLAIN) {
-            grid = NT / blockSize + (NT % blockSize == 0 ? 0 : 1);   // PT-IF:344
+            grid = NT / B + (NT % B == 0 ? 0 : 1);   // PT-IF:344
             return MPGOS_OK;
         }
         int perCU = 0;
-        MPGOS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)fn, blockSize, dyn_lds()));
+        MPGOS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)fn, B, dyn_lds()));
         if (perCU < 1) perCU = 1;
-        const int wavesPerBlock = (blockSize + 63) / 64;
+        const int wavesPerBlock = (B + 63) / 64;
         if (wavesPerSimd > 0) {
             const int want = (4 * wavesPerSimd + wavesPerBlock - 1) / wavesPerBlock;
             if (want < perCU) perCU = want;
         }
         grid = perCU * numCU;
-        const int needed = nActive / blockSize + (nActive % blockSize == 0 ? 0 : 1);
+        const int needed = nActive / B + (nActive % B == 0 ? 0 : 1);
         if (grid > needed) grid = needed;
         if (grid < 1) grid = 1;
         return MPGOS_OK;
@@ -497,7 +512,7 @@ public:
     {
         if (kernelMode == MPGOS_KERNEL_PERSISTENT)
             hipLaunchKernelGGL(reset_queue_kernel<0>, dim3(1), dim3(64), 0, strm, queue);
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(blockSize), dyn_lds(), strm, a);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(block()), dyn_lds(), strm, a);
         MPGOS_HIP_TRY(hipGetLastError());
         return MPGOS_OK;
     }
@@ -519,7 +534,7 @@ public:
             const unsigned char* ab = reinterpret_cast<const unsigned char*>(&a);
             const bool same = graphExec && key == graphKey && graphArgs.size() == sizeof a &&
                               std::memcmp(graphArgs.data(), ab, sizeof a) == 0 && graphGrid == grid &&
-                              graphBlock == blockSize;
+                              graphBlock == block();
             if (!same) {
                 if (graphExec) { (void)hipGraphExecDestroy(graphExec); graphExec = nullptr; }
                 if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
@@ -532,7 +547,7 @@ public:
                 graph = g;
                 MPGOS_HIP_TRY(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
                 graphArgs.assign(ab, ab + sizeof a);
-                graphKey = key; graphGrid = grid; graphBlock = blockSize;
+                graphKey = key; graphGrid = grid; graphBlock = block();
             }
             MPGOS_HIP_TRY(hipGraphLaunch(graphExec, strm));
         }

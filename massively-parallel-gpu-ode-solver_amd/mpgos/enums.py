@@ -59,6 +59,7 @@ class ListOfSolverOptions(enum.IntEnum):
     GraphCapture = 16
     EventLogCapacity = 17
     LaunchVariant = 18
+    PersistentRoundSteps = 19
 
 
 class KernelMode(enum.IntEnum):
