@@ -159,3 +159,20 @@ def test_fp32_modes_error_against_fp64_oracle(gpu, oracle, precision):
     out = run_gpu(spec, precision=precision)
     rel = np.abs(out["X"].astype(np.float64) - ref["X"]).max() / np.abs(ref["X"]).max()
     assert rel < 1e-3, rel
+
+
+def test_dropin_cpp_driver_matches_oracle(gpu, oracle, tmp_path):
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "lorenz_dropin")
+    assert os.path.exists(exe), "build() compiles tests/cpp/lorenz_dropin"
+    spec = W.config1_lorenz(nt=4096)
+    inp = np.concatenate([spec["cPAR"][0], spec["X"].ravel()]).astype(np.float64)
+    (tmp_path / "in.bin").write_bytes(inp.tobytes())
+    r = subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    out = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.float64).reshape(5, 4096)
+    ref = oracle.run("lorenz", spec)
+    assert np.array_equal(out[:3].view(np.uint64), ref["X"].view(np.uint64))
+    assert np.array_equal(out[3], ref["T"]) and np.array_equal(out[4].astype(np.int64), ref["nacc"])
