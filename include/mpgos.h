@@ -74,6 +74,8 @@ enum {
     MPGOS_PT_EVENT_COUNT,      /* int32 [NT]: detected events (all indices) in the last Solve */
     MPGOS_PT_EVENT_LOG,        /* P [NT*(1+SD)*cap]: (t, x_0..x_SD-1) per logged hit, SoA */
     MPGOS_PT_EVENT_LOG_INDEX,  /* int32 [NT*cap]: event function index of each logged hit */
+    MPGOS_PT_ADJOINT_STATE,    /* double [SD*NT]: adjoint input dL/dx(T); after mpgos_pt_adjoint_reverse, dL/dx(0) */
+    MPGOS_PT_PARAMETER_GRADIENT, /* double [NCP*NT]: dL/dcPAR after mpgos_pt_adjoint_reverse */
     MPGOS_PT_NUM_VARIABLES
 };
 
@@ -113,7 +115,8 @@ enum {
     MPGOS_STATUS_NONFINITE_RK4 = 1,      /* RK4 state not finite: instance terminated */
     MPGOS_STATUS_NONFINITE_MIN_STEP = 2, /* RKCK45 not finite even at minimum step: terminated */
     MPGOS_STATUS_MIN_STEP_REACHED = 4,   /* step forced at the minimum size, tolerance not guaranteed */
-    MPGOS_STATUS_EVENT_OMITTED = 8       /* event step below the minimum step: detection omitted */
+    MPGOS_STATUS_EVENT_OMITTED = 8,      /* event step below the minimum step: detection omitted */
+    MPGOS_STATUS_CHECKPOINT_OVERFLOW = 16 /* adjoint record pass: more checkpoints than the capacity */
 };
 
 typedef struct mpgos_solver mpgos_solver;
@@ -201,6 +204,21 @@ int mpgos_pt_last_solve_ms(const mpgos_solver* h, double* ms);
 
 /* HIP stream of the handle (hipStream_t), for callers that order their own work. */
 void* mpgos_pt_stream(mpgos_solver* h);
+
+/* Discrete adjoint (no counterpart in the reference; FP64, systems with a
+ * right-hand-side VJP).  The step sequence of the recorded Solve is taken as
+ * given (the controller is not differentiated).
+ *   mpgos_pt_adjoint_record: a Solve() that also stores (x, t, proposed dt)
+ *     before the first and after every `checkpoint_interval`-th accepted step,
+ *     at most `checkpoint_capacity` checkpoints per instance (exceeding it sets
+ *     MPGOS_STATUS_CHECKPOINT_OVERFLOW for that instance).
+ *   mpgos_pt_adjoint_reverse: reads MPGOS_PT_ADJOINT_STATE = dL/dx(T) from the
+ *     device buffer, recomputes each checkpoint segment with the same adaptive
+ *     loop and propagates the adjoint backwards through the Runge-Kutta stages;
+ *     leaves dL/dx(0) in MPGOS_PT_ADJOINT_STATE and dL/dcPAR in
+ *     MPGOS_PT_PARAMETER_GRADIENT (device buffers; synchronise to read). */
+int mpgos_pt_adjoint_record(mpgos_solver* h, int checkpoint_interval, int checkpoint_capacity);
+int mpgos_pt_adjoint_reverse(mpgos_solver* h);
 
 /* Names of the registered per-thread systems, comma separated. */
 const char* mpgos_pt_list_systems(void);

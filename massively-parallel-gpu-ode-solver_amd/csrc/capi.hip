@@ -155,6 +155,18 @@ void* mpgos_pt_stream(mpgos_solver* h)
     return h->pt->stream();
 }
 
+int mpgos_pt_adjoint_record(mpgos_solver* h, int checkpoint_interval, int checkpoint_capacity)
+{
+    MPGOS_CHECK_HANDLE(h);
+    return h->pt->adjoint_record(checkpoint_interval, checkpoint_capacity);
+}
+
+int mpgos_pt_adjoint_reverse(mpgos_solver* h)
+{
+    MPGOS_CHECK_HANDLE(h);
+    return h->pt->adjoint_reverse();
+}
+
 int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, long n)
 {
     if (fn < 0 || fn > 4 || !x || !out || (fn == 4 && !y)) { set_error("mpgos_math_eval_host: bad arguments"); return MPGOS_E_ARG; }

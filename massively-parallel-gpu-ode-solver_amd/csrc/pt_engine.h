@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "mpgos.h"
+#include "pt_adjoint.h"
 #include "pt_device.h"
 
 namespace mpgos {
@@ -64,8 +65,8 @@ public:
     virtual int print(int var, int tid, const char* dir) = 0;
     virtual int last_solve_ms(double* ms) const = 0;
     virtual void* stream() = 0;
-    // adjoint (pt_adjoint.h); default: unsupported
-    virtual int adjoint(const void*, void*, void*, int) { set_error("adjoint not supported by this system"); return MPGOS_E_SYSTEM; }
+    virtual int adjoint_record(int K, int capacity) = 0;
+    virtual int adjoint_reverse() = 0;
 };
 
 template <class T>
@@ -124,6 +125,11 @@ public:
     Buffer<P> TD, X, T, cPAR, sPAR, ACC, DOt, DOx, evlog;
     Buffer<int> sPARi, ACCi, DOidx, status, nev, evlogidx;
     Buffer<long long> nacc, nrej;
+    Buffer<P> lam, gp;                      // adjoint state and parameter gradient
+    double* ckpt = nullptr;                 // adjoint checkpoints (device only)
+    double* seg = nullptr;                  // adjoint segment buffer (device only)
+    size_t ckptCount = 0, segCount = 0;
+    int adjK = 0, adjCap = 0;
     unsigned int* queue = nullptr;
 
     // graph replay
@@ -140,7 +146,9 @@ public:
         TD.release(); X.release(); T.release(); cPAR.release(); sPAR.release(); ACC.release();
         DOt.release(); DOx.release(); evlog.release();
         sPARi.release(); ACCi.release(); DOidx.release(); status.release(); nev.release(); evlogidx.release();
-        nacc.release(); nrej.release();
+        nacc.release(); nrej.release(); lam.release(); gp.release();
+        if (ckpt) (void)hipFree(ckpt);
+        if (seg) (void)hipFree(seg);
         if (queue) (void)hipFree(queue);
         if (syncEvent) (void)hipEventDestroy(syncEvent);
         if (evStart) (void)hipEventDestroy(evStart);
@@ -169,7 +177,8 @@ public:
             (rc = cPAR.alloc(NCP * nt_)) || (rc = sPAR.alloc(NSP)) || (rc = sPARi.alloc(NISP)) ||
             (rc = ACC.alloc(NA * nt_)) || (rc = ACCi.alloc(NIA * nt_)) || (rc = DOidx.alloc(nt_)) ||
             (rc = DOt.alloc((size_t)ndo * nt_)) || (rc = DOx.alloc((size_t)SD * ndo * nt_)) ||
-            (rc = status.alloc(nt_)) || (rc = nev.alloc(nt_)) || (rc = nacc.alloc(nt_)) || (rc = nrej.alloc(nt_)))
+            (rc = status.alloc(nt_)) || (rc = nev.alloc(nt_)) || (rc = nacc.alloc(nt_)) || (rc = nrej.alloc(nt_)) ||
+            (rc = lam.alloc(SD * nt_)) || (rc = gp.alloc(NCP * nt_)))
             return rc;
         MPGOS_HIP_TRY(hipMalloc((void**)&queue, 256));
         rtol.assign(SD, 1e-8);
@@ -321,6 +330,14 @@ public:
             f(&evlogidx.h[(size_t)problem + (size_t)step * NT]);
             return MPGOS_OK;
         }
+        case MPGOS_PT_ADJOINT_STATE:
+            if ((rc = bound(fn, "ProblemNumber", problem, 0, NT - 1)) || (rc = bound(fn, "AdjointState", serial, 0, SD - 1))) return rc;
+            f(&lam.h[(size_t)problem + (size_t)serial * NT]);
+            return MPGOS_OK;
+        case MPGOS_PT_PARAMETER_GRADIENT:
+            if ((rc = bound(fn, "ProblemNumber", problem, 0, NT - 1)) || (rc = bound(fn, "ParameterGradient", serial, 0, NCP - 1))) return rc;
+            f(&gp.h[(size_t)problem + (size_t)serial * NT]);
+            return MPGOS_OK;
         default:
             set_error(std::string(fn) + ": variable not applicable with this argument configuration");
             return MPGOS_E_ARG;
@@ -333,7 +350,7 @@ public:
             // PT-IF:767-770: the value itself is bound-checked
             if (int rc = bound("SetHost", "DenseIndex-Value", (int)v, 0, NDO - 1)) return rc;
         }
-        if (var >= MPGOS_PT_ACCEPTED_STEPS) { set_error("SetHost: output-only variable"); return MPGOS_E_ARG; }
+        if (var >= MPGOS_PT_ACCEPTED_STEPS && var != MPGOS_PT_ADJOINT_STATE) { set_error("SetHost: output-only variable"); return MPGOS_E_ARG; }
         return access("SetHost", var, problem, serial, step, [&](auto* p) {
             using T_ = typename std::remove_pointer<decltype(p)>::type;
             *p = std::is_integral<T_>::value ? (T_)(int)v : (T_)v;
@@ -371,6 +388,8 @@ public:
         case MPGOS_PT_EVENT_COUNT: return ret(nev, host, nbytes);
         case MPGOS_PT_EVENT_LOG: return ret(evlog, host, nbytes);
         case MPGOS_PT_EVENT_LOG_INDEX: return ret(evlogidx, host, nbytes);
+        case MPGOS_PT_ADJOINT_STATE: return ret(lam, host, nbytes);
+        case MPGOS_PT_PARAMETER_GRADIENT: return ret(gp, host, nbytes);
         default:
             set_error("buffer: variable has no single buffer");
             if (nbytes) *nbytes = 0;
@@ -415,6 +434,10 @@ public:
                 if ((rc = copy(nacc, false)) || (rc = copy(nrej, false)) || (rc = copy(status, false)) || (rc = copy(nev, false))) return rc;
             }
             return MPGOS_OK;
+        case MPGOS_PT_ADJOINT_STATE: return copy(lam, to_device);
+        case MPGOS_PT_PARAMETER_GRADIENT:
+            if (to_device) { set_error("SynchroniseFromHostToDevice: output-only variable"); return MPGOS_E_ARG; }
+            return copy(gp, false);
         case MPGOS_PT_ACCEPTED_STEPS: case MPGOS_PT_REJECTED_STEPS: case MPGOS_PT_STATUS: case MPGOS_PT_EVENT_COUNT:
         case MPGOS_PT_EVENT_LOG: case MPGOS_PT_EVENT_LOG_INDEX:
             if (to_device) { set_error("SynchroniseFromHostToDevice: output-only variable"); return MPGOS_E_ARG; }
@@ -555,6 +578,79 @@ public:
         return MPGOS_OK;
     }
     int graphGrid = 0, graphBlock = 0;
+
+    // ------------------------------------------------------------ adjoint (pt_adjoint.h)
+    static constexpr bool kAdjoint = Sys::HAS_VJP && std::is_same<P, double>::value && !KAHAN;
+
+    int adjoint_args(AdjArgs<Sys>& A)
+    {
+        if constexpr (kAdjoint) {
+            A.a = make_args();
+            A.ckpt = ckpt; A.seg = seg; A.lam = lam.d; A.gp = gp.d; A.K = adjK; A.ckcap = adjCap;
+        }
+        return MPGOS_OK;
+    }
+
+    int adjoint_record(int K, int capacity) override
+    {
+        if constexpr (!kAdjoint) {
+            set_error("adjoint: needs FP64 and a system with a right-hand-side VJP");
+            return MPGOS_E_SYSTEM;
+        } else {
+            if (K < 1 || capacity < 1) { set_error("adjoint: checkpoint interval and capacity must be >= 1"); return MPGOS_E_ARG; }
+            MPGOS_HIP_TRY(hipSetDevice(device));
+            const size_t needC = (size_t)capacity * checkpoint_words<Sys>() * NT, needS = (size_t)K * (SD + 2) * NT;
+            if (needC > ckptCount || needS > segCount) MPGOS_HIP_TRY(hipStreamSynchronize(strm));
+            if (needC > ckptCount) {
+                if (ckpt) (void)hipFree(ckpt);
+                ckpt = nullptr; ckptCount = 0;
+                if (hipMalloc((void**)&ckpt, needC * sizeof(double)) != hipSuccess) { set_error("adjoint: checkpoint allocation failed"); return MPGOS_E_MEMORY; }
+                ckptCount = needC;
+            }
+            if (needS > segCount) {
+                if (seg) (void)hipFree(seg);
+                seg = nullptr; segCount = 0;
+                if (hipMalloc((void**)&seg, needS * sizeof(double)) != hipSuccess) { set_error("adjoint: segment allocation failed"); return MPGOS_E_MEMORY; }
+                segCount = needS;
+            }
+            adjK = K; adjCap = capacity;
+            AdjArgs<Sys> A;
+            adjoint_args(A);
+            const int B = blockSize;
+            const int grid = NT / B + (NT % B == 0 ? 0 : 1);
+            const size_t lds = preferShared ? (size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int) : 0;
+            MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((adjoint_record_kernel<Sys, MPGOS_RK4>), dim3(grid), dim3(B), lds, strm, A);
+            else hipLaunchKernelGGL((adjoint_record_kernel<Sys, MPGOS_RKCK45>), dim3(grid), dim3(B), lds, strm, A);
+            MPGOS_HIP_TRY(hipGetLastError());
+            MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
+            return MPGOS_OK;
+        }
+    }
+
+    int adjoint_reverse() override
+    {
+        if constexpr (!kAdjoint) {
+            set_error("adjoint: needs FP64 and a system with a right-hand-side VJP");
+            return MPGOS_E_SYSTEM;
+        } else {
+            if (!ckpt || adjK == 0) { set_error("adjoint: no recorded forward pass (call mpgos_pt_adjoint_record first)"); return MPGOS_E_STATE; }
+            MPGOS_HIP_TRY(hipSetDevice(device));
+            AdjArgs<Sys> A;
+            adjoint_args(A);
+            A.a.NDO = 0;       // the recomputation writes nothing but the segment buffer
+            A.a.evcap = 0;
+            const int B = blockSize;
+            const int grid = NT / B + (NT % B == 0 ? 0 : 1);
+            const size_t lds = preferShared ? (size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int) : 0;
+            MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((adjoint_reverse_kernel<Sys, MPGOS_RK4>), dim3(grid), dim3(B), lds, strm, A);
+            else hipLaunchKernelGGL((adjoint_reverse_kernel<Sys, MPGOS_RKCK45>), dim3(grid), dim3(B), lds, strm, A);
+            MPGOS_HIP_TRY(hipGetLastError());
+            MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
+            return MPGOS_OK;
+        }
+    }
 
     int synchronise_device() override { MPGOS_HIP_TRY(hipSetDevice(device)); MPGOS_HIP_TRY(hipDeviceSynchronize()); return MPGOS_OK; }
     int insert_sync_point() override { MPGOS_HIP_TRY(hipSetDevice(device)); MPGOS_HIP_TRY(hipEventRecord(syncEvent, strm)); return MPGOS_OK; }

@@ -52,6 +52,8 @@ def lib():
         "mpgos_pt_print": (c_int, [H, c_int, c_int, c_char_p]),
         "mpgos_pt_last_solve_ms": (c_int, [H, ctypes.POINTER(c_double)]),
         "mpgos_pt_stream": (c_void_p, [H]),
+        "mpgos_pt_adjoint_record": (c_int, [H, c_int, c_int]),
+        "mpgos_pt_adjoint_reverse": (c_int, [H]),
         "mpgos_math_eval_host": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_long]),
         "mpgos_math_eval_device": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_long]),
     }

@@ -36,6 +36,8 @@ class ListOfVariables(enum.IntEnum):
     EventCount = 16
     EventLog = 17
     EventLogIndex = 18
+    AdjointState = 19
+    ParameterGradient = 20
 
 
 class ListOfSolverOptions(enum.IntEnum):
@@ -72,6 +74,7 @@ class Status(enum.IntFlag):
     NONFINITE_MIN_STEP = 2
     MIN_STEP_REACHED = 4
     EVENT_OMITTED = 8
+    CHECKPOINT_OVERFLOW = 16
 
 
 # export the reference's bare enumerator names (RK4, TimeDomain, InitialTimeStep, ...)

@@ -132,6 +132,7 @@ class ProblemSolver:
             V.Status: (np.int32, (NT,)), V.EventCount: (np.int32, (NT,)),
             V.EventLog: (fp, (self.event_log_capacity, self.SD + 1, NT)),
             V.EventLogIndex: (np.int32, (self.event_log_capacity, NT)),
+            V.AdjointState: (fp, (self.SD, NT)), V.ParameterGradient: (fp, (self.NCP, NT)),
         }
         if var not in table:
             raise MPGOSError(f"no single buffer for {ListOfVariables(var).name}")
@@ -186,6 +187,31 @@ class ProblemSolver:
 
     def stream(self):
         return lib().mpgos_pt_stream(self._h)
+
+    # ---------------------------------------------------------------- discrete adjoint
+    def adjoint_record(self, checkpoint_interval, checkpoint_capacity):
+        """Solve() that also keeps (x, t, proposed dt) every `checkpoint_interval`
+        accepted steps (at most `checkpoint_capacity` per instance)."""
+        check(lib().mpgos_pt_adjoint_record(self._h, int(checkpoint_interval), int(checkpoint_capacity)))
+
+    def adjoint_reverse(self):
+        """Reverse sweep: AdjointState holds dL/dx(T) on entry (device buffer),
+        dL/dx(0) on exit; ParameterGradient receives dL/dcPAR."""
+        check(lib().mpgos_pt_adjoint_reverse(self._h))
+
+    def adjoint(self, lam_T, checkpoint_interval, checkpoint_capacity):
+        """Convenience: record, upload dL/dx(T) = lam_T (SD x NT), reverse, read
+        back.  Returns (dL/dx(0), dL/dcPAR) as numpy arrays; the forward result
+        of the recorded Solve is left in the solver's buffers."""
+        self.adjoint_record(checkpoint_interval, checkpoint_capacity)
+        self.host_array(V.AdjointState)[...] = lam_T
+        self.SynchroniseFromHostToDevice(V.AdjointState)
+        self.adjoint_reverse()
+        self.SynchroniseFromDeviceToHost(V.AdjointState)
+        self.SynchroniseFromDeviceToHost(V.ParameterGradient)
+        self.InsertSynchronisationPoint()
+        self.SynchroniseSolver()
+        return self.host_array(V.AdjointState).copy(), self.host_array(V.ParameterGradient).copy()
 
     def solve_blocking(self):
         """Solve, copy everything back and wait (the reference's usual sequence)."""

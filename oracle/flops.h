@@ -42,6 +42,7 @@ inline F fabs(F a) { return F(std::fabs(a.v)); }
 inline F fmin(F a, F b) { return F(std::fmin(a.v, b.v)); }
 inline F fmax(F a, F b) { return F(std::fmax(a.v, b.v)); }
 inline bool isfinite(F a) { return std::isfinite(a.v); }
+inline F frozen(F a) { return a; }
 F cos(F a);
 F sin(F a);
 F exp(F a);

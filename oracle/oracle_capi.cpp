@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "dual.h"
 #include "flops.h"
 #include "pt_oracle.h"
 #include "systems/affine2.h"
@@ -53,7 +54,32 @@ static void trace_one(const Job& j, int tid, typename Instance<Sys, double>::Tra
     inst.run();
 }
 
+// Tangent-linear derivatives of the final state with respect to the initial
+// state and the control parameters, on the frozen step sequence:
+// jac[(i * (SD + NCP) + j) * NT + tid] = d x_i(T) / d z_j, z = (x(0), cPAR).
+template <class Sys>
+static void sensitivity_all(const Job& j, double* jac)
+{
+    constexpr int N = Sys::SD + Sys::NCP;
+    using S = dual::D<N>;
+    std::vector<S> sp(Sys::NSP > 0 ? Sys::NSP : 1);
+    std::vector<int> spi(Sys::NISP > 0 ? Sys::NISP : 1);
+    for (int i = 0; i < Sys::NSP; i++) sp[i] = S(j.sPAR[i]);
+    for (int i = 0; i < Sys::NISP; i++) spi[i] = j.sPARi[i];
+    const int threads = j.nthreads > 0 ? j.nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+    for (int tid = 0; tid < j.n_active; tid++) {
+        std::vector<S> spl(sp);
+        std::vector<int> spil(spi);
+        Instance<Sys, S> inst(j, tid, spl.data(), spil.data());
+        inst.run();
+        for (int a = 0; a < Sys::SD; a++)
+            for (int b = 0; b < N; b++) jac[((size_t)a * N + b) * j.nt + tid] = inst.x[a].d[b];
+    }
+}
+
 typedef void (*Runner)(const Job&);
+typedef void (*Sensitivity)(const Job&, double*);
 struct TraceArgs { int cap; int n; double* t; double* h; int* acc; double* ratio; };
 typedef void (*Tracer)(const Job&, int, TraceArgs*);
 template <class Sys>
@@ -62,15 +88,15 @@ static void trace_entry(const Job& j, int tid, TraceArgs* a)
     static_assert(sizeof(TraceArgs) == sizeof(typename Instance<Sys, double>::Trace), "trace layout");
     trace_one<Sys>(j, tid, reinterpret_cast<typename Instance<Sys, double>::Trace*>(a));
 }
-struct Entry { const char* name; Runner run; Runner count; Tracer trace; };
+struct Entry { const char* name; Runner run; Runner count; Tracer trace; Sensitivity sens; };
 
 static const Entry kSystems[] = {
-    {"duffing", run_all<mpgos::systems::Duffing, double>, run_all<mpgos::systems::Duffing, flops::F>, trace_entry<mpgos::systems::Duffing>},
-    {"duffing_q", run_all<mpgos::systems::DuffingQ, double>, run_all<mpgos::systems::DuffingQ, flops::F>, trace_entry<mpgos::systems::DuffingQ>},
-    {"duffing_eps", run_all<mpgos::systems::DuffingEps, double>, run_all<mpgos::systems::DuffingEps, flops::F>, trace_entry<mpgos::systems::DuffingEps>},
-    {"duffing_dw", run_all<mpgos::systems::DuffingDW, double>, run_all<mpgos::systems::DuffingDW, flops::F>, trace_entry<mpgos::systems::DuffingDW>},
-    {"lorenz", run_all<mpgos::systems::Lorenz, double>, run_all<mpgos::systems::Lorenz, flops::F>, trace_entry<mpgos::systems::Lorenz>},
-    {"affine2", run_all<mpgos::systems::Affine2, double>, run_all<mpgos::systems::Affine2, flops::F>, trace_entry<mpgos::systems::Affine2>},
+    {"duffing", run_all<mpgos::systems::Duffing, double>, run_all<mpgos::systems::Duffing, flops::F>, trace_entry<mpgos::systems::Duffing>, sensitivity_all<mpgos::systems::Duffing>},
+    {"duffing_q", run_all<mpgos::systems::DuffingQ, double>, run_all<mpgos::systems::DuffingQ, flops::F>, trace_entry<mpgos::systems::DuffingQ>, sensitivity_all<mpgos::systems::DuffingQ>},
+    {"duffing_eps", run_all<mpgos::systems::DuffingEps, double>, run_all<mpgos::systems::DuffingEps, flops::F>, trace_entry<mpgos::systems::DuffingEps>, sensitivity_all<mpgos::systems::DuffingEps>},
+    {"duffing_dw", run_all<mpgos::systems::DuffingDW, double>, run_all<mpgos::systems::DuffingDW, flops::F>, trace_entry<mpgos::systems::DuffingDW>, sensitivity_all<mpgos::systems::DuffingDW>},
+    {"lorenz", run_all<mpgos::systems::Lorenz, double>, run_all<mpgos::systems::Lorenz, flops::F>, trace_entry<mpgos::systems::Lorenz>, sensitivity_all<mpgos::systems::Lorenz>},
+    {"affine2", run_all<mpgos::systems::Affine2, double>, run_all<mpgos::systems::Affine2, flops::F>, trace_entry<mpgos::systems::Affine2>, sensitivity_all<mpgos::systems::Affine2>},
 };
 
 static const Entry* find(const char* name)
@@ -109,6 +135,14 @@ int oracle_pt_count_flops(const char* system, oracle::Job* job, long long* flops
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+int oracle_pt_sensitivity(const char* system, oracle::Job* job, double* jac)
+{
+    const oracle::Entry* e = oracle::find(system);
+    if (!e || !job || !jac) return -4;
+    e->sens(*job, jac);
+    return 0;
+}
 
 // Per-attempt trace of instance `tid` (time before the step, step size,
 // accepted flag, tol/err ratio of the RKCK45 controller).  Returns the number

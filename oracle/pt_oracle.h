@@ -55,6 +55,10 @@ template <int N> struct AtLeast1 { static constexpr int v = N > 0 ? N : 1; };
 
 inline double dmin(double a, double b) { return std::fmin(a, b); }
 inline double dmax(double a, double b) { return std::fmax(a, b); }
+// Step sizes carry no derivative (dual.h overloads this for dual numbers).
+inline double frozen(double a) { return a; }
+// Seeds the differentiation directions after the inputs are read (dual.h).
+template <class S> inline void seed_directions(S*, int, S*, int) {}
 
 // The scalar type S is double for the oracle proper; the op-counting scalar
 // (flops.h) and the dual numbers (dual.h) reuse the same code.
@@ -86,6 +90,7 @@ struct Instance {
     static bool fin(const S& a) { using std::isfinite; return isfinite(a); }
     static S fm(const S& a, const S& b, const S& c) { using std::fma; return fma(a, b, c); }
     static S pw(const S& a, double e) { using mpgos::math::pow; return pow(a, S(e)); }
+    static S frz(const S& a) { using oracle::frozen; return frozen(a); }
 
     void read()
     {
@@ -235,7 +240,7 @@ struct Instance {
             flags |= ST_MIN_STEP;
         }
         factor = fmx(fmn(factor, S(J.grow)), S(J.shrink));
-        dtNew = fmx(fmn(dt * factor, S(J.dtmax)), S(J.dtmin));
+        dtNew = frz(fmx(fmn(dt * factor, S(J.dtmax)), S(J.dtmin)));
     }
 
     static bool crosses(const S& a, const S& b, double tol, int dir)
@@ -250,6 +255,7 @@ struct Instance {
     void run()
     {
         read();
+        seed_directions(x, SD, cp, NCP);
         dt = S(J.dt0);
         dtNew = S(J.dt0);
         doSkipped = 0;
@@ -293,7 +299,7 @@ struct Instance {
                 }
                 if (shorten) {
                     if (dtEvent < S(J.dtmin)) flags |= ST_EVENT_OMITTED;
-                    else { dtNew = dtEvent; accept = 0; }
+                    else { dtNew = frz(dtEvent); accept = 0; }
                 }
             }
 

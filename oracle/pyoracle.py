@@ -157,3 +157,24 @@ def trace(system, spec, tid=0, cap=100000):
     run(system, spec, _job_hook=hook)
     k = holder["n"]
     return {"t": t[:k], "h": h[:k], "accepted": acc[:k], "ratio": ratio[:k]}
+
+
+def sensitivity(system, spec, nthreads=0):
+    """d x(T) / d (x(0), cPAR) on the frozen accepted-step sequence, shape
+    (SD, SD + NCP, NT), plus the ordinary outputs of the same run."""
+    L = lib()
+    if not hasattr(L, "_sens_set"):
+        L.oracle_pt_sensitivity.restype = ctypes.c_int
+        L.oracle_pt_sensitivity.argtypes = [ctypes.c_char_p, ctypes.POINTER(Job), _dp]
+        L._sens_set = True
+    sd, ncp, nt = spec["sd"], spec["ncp"], spec["nt"]
+    jac = np.zeros((sd, sd + ncp, nt))
+
+    def hook(j):
+        j.nthreads = int(nthreads)
+        if L.oracle_pt_sensitivity(system.encode(), ctypes.byref(j), _ptr(jac, _dp)) != 0:
+            raise RuntimeError("oracle sensitivity failed")
+
+    out = run(system, spec, _job_hook=hook)
+    out["jac"] = jac
+    return out

@@ -1,0 +1,146 @@
+"""Discrete adjoint on the GPU (FP64) against the oracle's forward
+sensitivities on the same frozen step sequence, and against central finite
+differences where the map is smooth (fixed steps)."""
+import numpy as np
+import pytest
+
+import scenarios as S
+from mpgos import Algorithms, ListOfVariables as V, ProblemSolver
+from mpgos import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_adjoint(spec, lam_T, K, cap=None):
+    s = ProblemSolver(spec["system"], spec["nt"], spec["algorithm"])
+    try:
+        W.load(s, spec)
+        if cap is None:
+            s.solve_blocking()
+            cap = int(s.host_array(V.AcceptedSteps).max()) // K + 2
+            W.load(s, spec)
+        lam0, gp = s.adjoint(lam_T, K, cap)
+        s.SynchroniseFromDeviceToHost(V.All)
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
+        out = W.read(s)
+    finally:
+        s.close()
+    return lam0, gp, out
+
+
+def reference_gradient(oracle, spec, lam_T):
+    o = oracle.sensitivity(spec["system"], spec)
+    jac = o["jac"]                                      # (SD, SD + NCP, NT)
+    g = np.einsum("in,ijn->jn", lam_T, jac)
+    return g[:spec["sd"]], g[spec["sd"]:], o
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("algorithm", [Algorithms.RK4, Algorithms.RKCK45])
+def test_duffing_q_adjoint_matches_forward_sensitivity(gpu, oracle, algorithm):
+    spec = W.config3_duffing_adjoint(nt=512, algorithm=algorithm)
+    spec["TD"][1] = 4 * (2 * W.PI)
+    rng = np.random.default_rng(3)
+    lam_T = rng.normal(size=(3, 512))
+    lam0, gp, out = gpu_adjoint(spec, lam_T, K=16)
+    rx, rp, ref = reference_gradient(oracle, spec, lam_T)
+    assert np.array_equal(out["X"].view(np.uint64), ref["X"].view(np.uint64))   # recorded forward = oracle
+    assert rel(lam0, rx) < 1e-9, rel(lam0, rx)
+    assert rel(gp, rp) < 1e-9, rel(gp, rp)
+
+
+def test_adjoint_independent_of_checkpoint_interval(gpu, oracle):
+    spec = W.config3_duffing_adjoint(nt=256, algorithm=Algorithms.RKCK45)
+    spec["TD"][1] = 2 * (2 * W.PI)
+    lam_T = np.ones((3, 256))
+    res = [gpu_adjoint(spec, lam_T, K=K)[:2] for K in (1, 7, 64)]
+    for r in res[1:]:
+        assert np.array_equal(r[0], res[0][0]) and np.array_equal(r[1], res[0][1])
+
+
+def test_lorenz_adjoint(gpu, oracle):
+    spec = W.config1_lorenz(nt=512)
+    spec["TD"][1] = 1.0
+    lam_T = np.random.default_rng(5).normal(size=(3, 512))
+    lam0, gp, _ = gpu_adjoint(spec, lam_T, K=32)
+    rx, rp, _ = reference_gradient(oracle, spec, lam_T)
+    assert rel(lam0, rx) < 1e-9 and rel(gp, rp) < 1e-9
+
+
+def test_affine_adjoint_closed_form(gpu, oracle):
+    A = np.array([[-0.5, 1.0], [-1.0, -0.1]])
+    spec = S.affine(nt=64, a00=A[0, 0], a01=A[0, 1], a10=A[1, 0], a11=A[1, 1], b0=0.3, b1=-0.2, x0=(1.0, 0.5),
+                    td=(0.0, 2.0), dt0=0.125, algorithm=Algorithms.RK4)
+    lam_T = np.tile(np.array([[1.0], [-2.0]]), (1, 64))
+    lam0, gp, _ = gpu_adjoint(spec, lam_T, K=4)
+    hA = 0.125 * A
+    M = np.eye(2) + hA + hA @ hA / 2 + hA @ hA @ hA / 6 + hA @ hA @ hA @ hA / 24
+    expect = np.linalg.matrix_power(M, 16).T @ lam_T[:, 0]
+    assert np.allclose(lam0[:, 0], expect, rtol=1e-13)
+
+
+def test_finite_differences_rk4(gpu, oracle):
+    # central differences (h = 1e-6, relative) of L = x(T) . lam on the RK4 map
+    spec = W.config3_duffing_adjoint(nt=64, algorithm=Algorithms.RK4)
+    spec["TD"][1] = 2 * (2 * W.PI)
+    lam_T = np.array([[1.0], [0.5], [1.0]]) * np.ones((3, 64))
+    lam0, gp, out = gpu_adjoint(spec, lam_T, K=32)
+    L = lambda o: (lam_T * o["X"]).sum(0)
+    for j, (key, row) in enumerate((("cPAR", 0), ("cPAR", 1), ("X", 0), ("X", 1))):
+        hstep = 1e-6 * np.abs(spec[key][row]).max()
+        sp, sm = dict(spec), dict(spec)
+        sp[key] = spec[key].copy(); sm[key] = spec[key].copy()
+        sp[key][row] += hstep
+        sm[key][row] -= hstep
+        fd = (L(oracle.run("duffing_q", sp)) - L(oracle.run("duffing_q", sm))) / (2 * hstep)
+        ad = gp[row] if key == "cPAR" else lam0[row]
+        assert rel(ad, fd) < 1e-6, (key, row, rel(ad, fd))
+
+
+def test_checkpoint_overflow_is_reported(gpu, oracle):
+    spec = W.config3_duffing_adjoint(nt=64, algorithm=Algorithms.RK4)
+    spec["TD"][1] = 2 * W.PI
+    lam0, gp, out = gpu_adjoint(spec, np.ones((3, 64)), K=4, cap=3)
+    assert (out["status"] & 16).all() and np.isnan(gp).all()
+
+
+@pytest.mark.parametrize("algorithm", [Algorithms.RK4, Algorithms.RKCK45])
+def test_torch_autograd_two_segment_loss(gpu, oracle, algorithm):
+    # the adjoint-configuration loss: mean of x^2 over the last forcing period
+    import torch
+    from adjoint_torch import duffing_period_loss
+    nt, periods = 128, 4
+    spec = W.config3_duffing_adjoint(nt=nt, algorithm=algorithm, periods=periods)
+    opts = {} if algorithm == Algorithms.RK4 else {"rtol": 1e-9, "atol": 1e-9}
+    if algorithm == Algorithms.RK4:
+        opts["options"] = {"dt0": 2 * W.PI / 500}
+    k = torch.tensor(spec["cPAR"][0], requires_grad=True)
+    B = torch.tensor(spec["cPAR"][1], requires_grad=True)
+    x0 = torch.tensor(spec["X"][0], requires_grad=True)
+    v0 = torch.tensor(spec["X"][1], requires_grad=True)
+    loss = duffing_period_loss(k, B, x0, v0, algorithm=algorithm, periods=periods, K=32, opts=opts)
+    loss.sum().backward()
+    # reference: compose the oracle's sensitivities of the two segments
+    two_pi = 2 * W.PI
+    s1 = dict(spec)
+    o1 = oracle.sensitivity("duffing_q", s1)
+    s2 = W.new_spec("duffing_q", algorithm, nt)
+    s2["cPAR"] = spec["cPAR"].copy()
+    s2["X"][0], s2["X"][1] = o1["X"][0], o1["X"][1]
+    s2["TD"][0], s2["TD"][1] = (periods - 1) * two_pi, periods * two_pi
+    s2["T"][:] = s2["TD"][0]
+    for key in ("rtol", "atol"):
+        s2[key] = spec[key]
+    s2["options"] = dict(spec["options"])
+    o2 = oracle.sensitivity("duffing_q", s2)
+    assert np.allclose(loss.detach().numpy(), o2["X"][2] / two_pi, rtol=1e-14, atol=0)
+    J1, J2 = o1["jac"], o2["jac"]                  # (3, 5, nt): z = (x, v, q, k, B)
+    dq_dxv = J2[2, :2]                             # (2, nt)
+    dL_dp = (J2[2, 3:] + np.einsum("an,apn->pn", dq_dxv, J1[:2, 3:])) / two_pi
+    dL_dx0 = np.einsum("an,apn->pn", dq_dxv, J1[:2, :2]) / two_pi
+    assert rel(k.grad.numpy(), dL_dp[0]) < 1e-6 and rel(B.grad.numpy(), dL_dp[1]) < 1e-6
+    assert rel(x0.grad.numpy(), dL_dx0[0]) < 1e-6 and rel(v0.grad.numpy(), dL_dx0[1]) < 1e-6

@@ -258,3 +258,32 @@ def test_tutorial_callbacks_semantics(oracle):
                     events=lambda t, y: y[0])
     assert o["ACCi"][0, 0] == len(ref.t_events[0])
     assert np.abs(o["X"][:, 0] - ref.y[:, -1]).max() < 1e-7
+
+
+# ------------------------------------------------------------------ forward sensitivities (adjoint reference)
+def test_sensitivity_of_rk4_on_a_linear_system_is_the_matrix_power(oracle):
+    # x' = A x + b with exact step grid h = 1/8 over [0, 2]: the RK4 map is
+    # x <- M x + N b, M = I + hA + (hA)^2/2 + (hA)^3/6 + (hA)^4/24, so
+    # d x(T)/d x(0) = M^16 and d x(T)/d b = sum_k M^k N
+    A = np.array([[-0.5, 1.0], [-1.0, -0.1]])
+    b = np.array([0.3, -0.2])
+    spec = S.affine(nt=1, a00=A[0, 0], a01=A[0, 1], a10=A[1, 0], a11=A[1, 1], b0=b[0], b1=b[1], x0=(1.0, 0.5),
+                    td=(0.0, 2.0), dt0=0.125, algorithm=Algorithms.RK4)
+    o = oracle.sensitivity("affine2", spec)
+    h = 0.125
+    hA = h * A
+    I = np.eye(2)
+    M = I + hA + hA @ hA / 2 + hA @ hA @ hA / 6 + hA @ hA @ hA @ hA / 24
+    N = h * (I + hA / 2 + hA @ hA / 6 + hA @ hA @ hA / 24)
+    J = o["jac"][:, :, 0]
+    assert np.allclose(J[:, :2], np.linalg.matrix_power(M, 16), rtol=1e-13, atol=1e-14)
+    G = sum(np.linalg.matrix_power(M, k) for k in range(16)) @ N
+    assert np.allclose(J[:, 6:8], G, rtol=1e-12, atol=1e-14)
+    # d x(T)/d a00 by central differences of the oracle itself (smooth map: fixed step)
+    eps = 1e-6
+    xs = []
+    for d in (eps, -eps):
+        sp = S.affine(nt=1, a00=A[0, 0] + d, a01=A[0, 1], a10=A[1, 0], a11=A[1, 1], b0=b[0], b1=b[1], x0=(1.0, 0.5),
+                      td=(0.0, 2.0), dt0=0.125, algorithm=Algorithms.RK4)
+        xs.append(oracle.run("affine2", sp)["X"][:, 0])
+    assert np.allclose((xs[0] - xs[1]) / (2 * eps), J[:, 2], rtol=1e-8, atol=1e-10)
