@@ -65,6 +65,26 @@ def lib():
     return L
 
 
+def hip_runtime_path():
+    """Path of the HIP runtime libmpgos.so is linked against (after lib() has
+    loaded it).  Other packages may bring their own copy of libamdhip64 into
+    the process; calls on buffers of this library must go to this one."""
+    lib()
+    candidates = []
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 6 and "libamdhip64" in parts[-1]:
+                    candidates.append(parts[-1])
+    except OSError:
+        pass
+    for p in candidates:
+        if p.startswith("/opt/rocm"):
+            return p
+    return candidates[0] if candidates else "libamdhip64.so.7"
+
+
 def check(rc):
     if rc != 0:
         msg = lib().mpgos_last_error().decode()

@@ -22,6 +22,7 @@ import time
 
 import numpy as np
 
+from ._lib import hip_runtime_path
 from .enums import ListOfSolverOptions as O, ListOfVariables as V
 
 _HIP = None
@@ -30,11 +31,12 @@ _HIP = None
 def hip():
     global _HIP
     if _HIP is None:
-        h = ctypes.CDLL("libamdhip64.so")
+        h = ctypes.CDLL(hip_runtime_path())
         h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         h.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
         h.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
         h.hipFree.argtypes = [ctypes.c_void_p]
+        h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         _HIP = h
     return _HIP
 

@@ -113,10 +113,8 @@ def test_repeated_solves_continue_the_integration(gpu, oracle):
 def test_device_math_matches_host_bitwise(gpu):
     import ctypes
     from test_math import FN, host_eval
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-    hip.hipFree.argtypes = [ctypes.c_void_p]
+    from mpgos.tuning import hip as hip_runtime
+    hip = hip_runtime()
     rng = np.random.default_rng(7)
     x = np.concatenate([rng.uniform(-3e4, 3e4, 100000), rng.uniform(-5, 5, 100000), 10.0 ** rng.uniform(-300, 300, 1000)])
     y = rng.uniform(0.1, 0.3, x.size)
