@@ -226,6 +226,101 @@ const char* mpgos_pt_list_systems(void);
 /* Message of the last failed call on this thread. */
 const char* mpgos_last_error(void);
 
+/* ---------------------------------------------------------- per-block module */
+
+/* Per-block ListOfVariables, same names and order as
+ * SourceCodes/CoupledSystems_PerBlock_Interface.cuh:35-53; entries from
+ * PB_ACCEPTED_STEPS on are new per-system outputs. */
+enum {
+    MPGOS_PB_ALL = 0,
+    MPGOS_PB_TIME_DOMAIN,
+    MPGOS_PB_ACTUAL_TIME,
+    MPGOS_PB_ACTUAL_STATE,
+    MPGOS_PB_UNIT_PARAMETERS,
+    MPGOS_PB_SYSTEM_PARAMETERS,
+    MPGOS_PB_GLOBAL_PARAMETERS,
+    MPGOS_PB_INTEGER_GLOBAL_PARAMETERS,
+    MPGOS_PB_UNIT_ACCESSORIES,
+    MPGOS_PB_INTEGER_UNIT_ACCESSORIES,
+    MPGOS_PB_SYSTEM_ACCESSORIES,
+    MPGOS_PB_INTEGER_SYSTEM_ACCESSORIES,
+    MPGOS_PB_COUPLING_MATRIX,
+    MPGOS_PB_COUPLING_STRENGTH,
+    MPGOS_PB_COUPLING_INDEX,
+    MPGOS_PB_DENSE_OUTPUT,
+    MPGOS_PB_DENSE_INDEX,
+    MPGOS_PB_DENSE_TIME,
+    MPGOS_PB_DENSE_STATE,
+    /* new */
+    MPGOS_PB_ACCEPTED_STEPS,   /* int64 [NS] */
+    MPGOS_PB_REJECTED_STEPS,   /* int64 [NS] */
+    MPGOS_PB_STATUS,           /* int32 [NS]: MPGOS_STATUS_* bits */
+    MPGOS_PB_NUM_VARIABLES
+};
+
+/* Per-block ListOfSolverOptions, same names and order as
+ * SourceCodes/CoupledSystems_PerBlock_Interface.cuh:55-68. */
+enum {
+    MPGOS_PB_INITIAL_TIME_STEP = 0,
+    MPGOS_PB_ACTIVE_SYSTEMS,
+    MPGOS_PB_MAXIMUM_TIME_STEP,
+    MPGOS_PB_MINIMUM_TIME_STEP,
+    MPGOS_PB_TIME_STEP_GROW_LIMIT,
+    MPGOS_PB_TIME_STEP_SHRINK_LIMIT,
+    MPGOS_PB_RELATIVE_TOLERANCE,
+    MPGOS_PB_ABSOLUTE_TOLERANCE,
+    MPGOS_PB_EVENT_TOLERANCE,
+    MPGOS_PB_EVENT_DIRECTION,
+    MPGOS_PB_DENSE_OUTPUT_SAVE_FREQUENCY,
+    MPGOS_PB_DENSE_OUTPUT_MINIMUM_TIME_STEP,
+    MPGOS_PB_SHARED_GLOBAL_VARIABLES,
+    MPGOS_PB_SHARED_COUPLING_MATRICES,
+    MPGOS_PB_NUM_OPTIONS
+};
+
+typedef struct {
+    int ns, ups, ud, spb, nc, cbw, cci;
+    int nup, nsp, ngp, nigp, nua, niua, nsa, nisa, ne, ndo;
+    int algorithm, precision;
+} mpgos_pb_dims;
+
+/* ProblemSolver<NS,UPS,UD,TPB,SPB,NC,CBW,CCI,...>::ProblemSolver(int)
+ *   (CoupledSystems_PerBlock_Interface.cuh:353-657).  UD, NC and the parameter
+ *   and accessory counts come from the registered system; NS, UPS, SPB
+ *   (systems per workgroup, SPB*UPS <= 1024), the coupling storage form
+ *   (CBW bandwidth radius, 0 = full; CCI circularity) and NDO are chosen here.
+ *   Defaults as the reference: dt0 1e-2, dtmax 1e6, dtmin 1e-12, grow 5,
+ *   shrink 0.1, rtol = atol = 1e-8, SharedGlobalVariables on. */
+int mpgos_pb_create(const char* system, int algorithm, int precision, int ns, int ups, int spb, int cbw, int cci,
+                    int ndo, int device, mpgos_solver** out);
+int mpgos_pb_destroy(mpgos_solver* h);
+int mpgos_pb_get_dims(const mpgos_solver* h, mpgos_pb_dims* out);
+/* SolverOption (CoupledSystems_PerBlock_Interface.cuh:772-865) */
+int mpgos_pb_solver_option(mpgos_solver* h, int option, int index, double value);
+/* SetHost / GetHost (CoupledSystems_PerBlock_Interface.cuh:868-1140, 1339-1529).
+ * Arguments by scope:
+ *   unit scope (ActualState, UnitParameters, [Integer]UnitAccessories): a0 system, a1 unit, a2 serial;
+ *   system scope (TimeDomain, SystemParameters, [Integer]SystemAccessories, DenseTime, CouplingStrength): a0 system, a2 serial;
+ *   global scope ([Integer]GlobalParameters, CouplingIndex): a2 serial;
+ *   CouplingMatrix: a0 coupling number, a1 row, a2 column (stored in the form given by CBW/CCI, PB-IF:994-1081);
+ *   DenseIndex, ActualTime, AcceptedSteps, RejectedSteps, Status: a0 system;
+ *   DenseState: a0 system, a1 unit, a2 component, a3 time-step number. */
+int mpgos_pb_set_host(mpgos_solver* h, int variable, int a0, int a1, int a2, int a3, double value);
+int mpgos_pb_get_host(const mpgos_solver* h, int variable, int a0, int a1, int a2, int a3, double* value);
+void* mpgos_pb_host_buffer(mpgos_solver* h, int variable, size_t* nbytes);
+void* mpgos_pb_device_buffer(mpgos_solver* h, int variable, size_t* nbytes);
+/* SynchroniseFromHostToDevice / FromDeviceToHost (PB-IF:1143-1337) */
+int mpgos_pb_sync_h2d(mpgos_solver* h, int variable);
+int mpgos_pb_sync_d2h(mpgos_solver* h, int variable);
+/* Solve (PB-IF:1949-1965), synchronisation (PB-IF:1967-1992) */
+int mpgos_pb_solve(mpgos_solver* h);
+int mpgos_pb_synchronise_device(mpgos_solver* h);
+int mpgos_pb_insert_synchronisation_point(mpgos_solver* h);
+int mpgos_pb_synchronise_solver(mpgos_solver* h);
+int mpgos_pb_last_solve_ms(const mpgos_solver* h, double* ms);
+void* mpgos_pb_stream(mpgos_solver* h);
+const char* mpgos_pb_list_systems(void);
+
 /* ---------------------------------------------------------- math probes
  * Evaluate the library's elementary functions (0 cos, 1 sin, 2 exp, 3 log,
  * 4 pow(x, y)) on the host, or on the device over device buffers, so tests can

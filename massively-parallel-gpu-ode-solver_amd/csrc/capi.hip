@@ -24,6 +24,14 @@ static const Registration kSystems[] = {
 };
 } // namespace pt
 
+namespace pb {
+static const Registration kSystems[] = {
+    {"ring_duffing", make_pb_ring_duffing},
+    {"ring_dw", make_pb_ring_dw},
+    {"pb_linear", make_pb_pb_linear},
+};
+} // namespace pb
+
 // Elementary-function probe kernel: same source as the solver's functions.
 __global__ void math_eval_kernel(int fn, const double* x, const double* y, double* out, long n)
 {
@@ -41,7 +49,8 @@ __global__ void math_eval_kernel(int fn, const double* x, const double* y, doubl
 } // namespace mpgos
 
 struct mpgos_solver {
-    mpgos::pt::SolverBase* pt = nullptr;
+    mpgos::pt::SolverBase* pt = nullptr;   // per-thread module
+    mpgos::pb::SolverBase* pb = nullptr;   // per-block module
 };
 
 using mpgos::set_error;
@@ -165,6 +174,91 @@ int mpgos_pt_adjoint_reverse(mpgos_solver* h)
 {
     MPGOS_CHECK_HANDLE(h);
     return h->pt->adjoint_reverse();
+}
+
+// ---------------------------------------------------------------- per-block
+#define MPGOS_CHECK_PB(h)                                                                         \
+    do {                                                                                          \
+        if (!(h) || !(h)->pb) { set_error("null or non-per-block solver handle"); return MPGOS_E_ARG; } \
+    } while (0)
+
+const char* mpgos_pb_list_systems(void)
+{
+    static std::string names;
+    if (names.empty()) {
+        for (const auto& r : mpgos::pb::kSystems) {
+            if (!names.empty()) names += ",";
+            names += r.name;
+        }
+    }
+    return names.c_str();
+}
+
+int mpgos_pb_create(const char* system, int algorithm, int precision, int ns, int ups, int spb, int cbw, int cci,
+                    int ndo, int device, mpgos_solver** out)
+{
+    if (!out || !system) { set_error("mpgos_pb_create: null argument"); return MPGOS_E_ARG; }
+    *out = nullptr;
+    for (const auto& r : mpgos::pb::kSystems) {
+        if (std::strcmp(r.name, system) != 0) continue;
+        mpgos::pb::SolverBase* s = r.make(precision);
+        if (!s) { set_error("mpgos_pb_create: unsupported precision"); return MPGOS_E_SYSTEM; }
+        int rc = s->init(algorithm, ns, ups, spb, cbw, cci, ndo, device);
+        if (rc != MPGOS_OK) { delete s; return rc; }
+        mpgos_solver* h = new mpgos_solver;
+        h->pb = s;
+        *out = h;
+        return MPGOS_OK;
+    }
+    set_error(std::string("mpgos_pb_create: unknown system '") + system + "' (registered: " + mpgos_pb_list_systems() + ")");
+    return MPGOS_E_SYSTEM;
+}
+
+int mpgos_pb_destroy(mpgos_solver* h) { MPGOS_CHECK_PB(h); delete h->pb; delete h; return MPGOS_OK; }
+int mpgos_pb_get_dims(const mpgos_solver* h, mpgos_pb_dims* out)
+{
+    MPGOS_CHECK_PB(h);
+    if (!out) { set_error("null dims pointer"); return MPGOS_E_ARG; }
+    return h->pb->dims(out);
+}
+int mpgos_pb_solver_option(mpgos_solver* h, int option, int index, double value) { MPGOS_CHECK_PB(h); return h->pb->option(option, index, value); }
+int mpgos_pb_set_host(mpgos_solver* h, int variable, int a0, int a1, int a2, int a3, double value)
+{
+    MPGOS_CHECK_PB(h);
+    return h->pb->set_host(variable, a0, a1, a2, a3, value);
+}
+int mpgos_pb_get_host(const mpgos_solver* h, int variable, int a0, int a1, int a2, int a3, double* value)
+{
+    MPGOS_CHECK_PB(h);
+    if (!value) { set_error("null value pointer"); return MPGOS_E_ARG; }
+    return h->pb->get_host(variable, a0, a1, a2, a3, value);
+}
+void* mpgos_pb_host_buffer(mpgos_solver* h, int variable, size_t* nbytes)
+{
+    if (!h || !h->pb) { set_error("null or non-per-block solver handle"); return nullptr; }
+    return h->pb->buffer(variable, true, nbytes);
+}
+void* mpgos_pb_device_buffer(mpgos_solver* h, int variable, size_t* nbytes)
+{
+    if (!h || !h->pb) { set_error("null or non-per-block solver handle"); return nullptr; }
+    return h->pb->buffer(variable, false, nbytes);
+}
+int mpgos_pb_sync_h2d(mpgos_solver* h, int variable) { MPGOS_CHECK_PB(h); return h->pb->sync(variable, true); }
+int mpgos_pb_sync_d2h(mpgos_solver* h, int variable) { MPGOS_CHECK_PB(h); return h->pb->sync(variable, false); }
+int mpgos_pb_solve(mpgos_solver* h) { MPGOS_CHECK_PB(h); return h->pb->solve(); }
+int mpgos_pb_synchronise_device(mpgos_solver* h) { MPGOS_CHECK_PB(h); return h->pb->synchronise_device(); }
+int mpgos_pb_insert_synchronisation_point(mpgos_solver* h) { MPGOS_CHECK_PB(h); return h->pb->insert_sync_point(); }
+int mpgos_pb_synchronise_solver(mpgos_solver* h) { MPGOS_CHECK_PB(h); return h->pb->synchronise_solver(); }
+int mpgos_pb_last_solve_ms(const mpgos_solver* h, double* ms)
+{
+    MPGOS_CHECK_PB(h);
+    if (!ms) { set_error("null pointer"); return MPGOS_E_ARG; }
+    return h->pb->last_solve_ms(ms);
+}
+void* mpgos_pb_stream(mpgos_solver* h)
+{
+    if (!h || !h->pb) { set_error("null or non-per-block solver handle"); return nullptr; }
+    return h->pb->stream();
 }
 
 int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, long n)

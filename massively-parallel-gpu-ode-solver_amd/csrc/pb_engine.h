@@ -1,0 +1,420 @@
+// Host side of the per-block module: buffers, options, Set/GetHost with the
+// reference's scopes and coupling-matrix storage forms, transfers and launches
+// of pb_device.h.  Implements the member functions of the reference's
+// per-block ProblemSolver (SourceCodes/CoupledSystems_PerBlock_Interface.cuh:
+// constructor 353-657, SolverOption 772-865, SetHost 868-1140 with the
+// coupling-matrix mapping 994-1081, Synchronise* 1143-1337, GetHost 1339-1529,
+// Solve 1949-1965) with return codes instead of exit().
+//
+// Unit-scope layout: (system * UPS + unit) + component * NS*UPS (the reference
+// pads each block's logical threads; the padding is an artefact of its launch
+// shape and is not visible through SetHost/GetHost, so it is not kept).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "mpgos.h"
+#include "pb_device.h"
+#include "pt_engine.h"   // Buffer, set_error, MPGOS_HIP_TRY
+
+namespace mpgos {
+namespace pb {
+
+using pt::Buffer;
+
+class SolverBase {
+public:
+    virtual ~SolverBase() {}
+    virtual int init(int algorithm, int ns, int ups, int spb, int cbw, int cci, int ndo, int device) = 0;
+    virtual int dims(mpgos_pb_dims* d) const = 0;
+    virtual int option(int opt, int idx, double v) = 0;
+    virtual int set_host(int var, int a0, int a1, int a2, int a3, double v) = 0;
+    virtual int get_host(int var, int a0, int a1, int a2, int a3, double* v) const = 0;
+    virtual void* buffer(int var, bool host, size_t* nbytes) = 0;
+    virtual int sync(int var, bool to_device) = 0;
+    virtual int solve() = 0;
+    virtual int synchronise_device() = 0;
+    virtual int insert_sync_point() = 0;
+    virtual int synchronise_solver() = 0;
+    virtual int last_solve_ms(double* ms) const = 0;
+    virtual void* stream() = 0;
+};
+
+template <class Sys, class P>
+class Solver : public SolverBase {
+public:
+    static constexpr int UD = Sys::UD, NC = Sys::NC, NUP = Sys::NUP, NSP = Sys::NSP, NGP = Sys::NGP, NiGP = Sys::NiGP;
+    static constexpr int NUA = Sys::NUA, NiUA = Sys::NiUA, NSA = Sys::NSA, NiSA = Sys::NiSA, NE = Sys::NE;
+
+    int algorithm = MPGOS_RKCK45, NS = 0, UPS = 0, SPB = 1, CBW = 0, CCI = 0, NDO = 0, device = 0;
+    int cMatSize = 0;   // elements per coupling matrix in the stored form
+    hipStream_t strm = nullptr;
+    hipEvent_t syncEvent = nullptr, evStart = nullptr, evStop = nullptr;
+    size_t ldsMax = 0;
+
+    int nActive = 0, sharedGlobals = 1, sharedMatrix = 0, doFreq = 1;
+    double dt0 = 1e-2, dtmax = 1.0e6, dtmin = 1.0e-12, grow = 5.0, shrink = 0.1, doMinDt = 0.0;
+    std::vector<double> rtol, atol, etol;
+    std::vector<int> edir;
+
+    Buffer<P> X, uPAR, uACC, TD, T, sPAR, sACC, strength, DOt, DOx, gPAR, cMat;
+    Buffer<int> uACCi, sACCi, DOidx, gPARi, cIndex, status;
+    Buffer<long long> nacc, nrej;
+    std::vector<P> cMatOriginal;   // the full matrices as set (GetHost), NC * UPS * UPS
+
+    ~Solver() override
+    {
+        if (strm) (void)hipStreamSynchronize(strm);
+        for (auto* b : {&X, &uPAR, &uACC, &TD, &T, &sPAR, &sACC, &strength, &DOt, &DOx, &gPAR, &cMat}) b->release();
+        for (auto* b : {&uACCi, &sACCi, &DOidx, &gPARi, &cIndex, &status}) b->release();
+        nacc.release(); nrej.release();
+        if (syncEvent) (void)hipEventDestroy(syncEvent);
+        if (evStart) (void)hipEventDestroy(evStart);
+        if (evStop) (void)hipEventDestroy(evStop);
+        if (strm) (void)hipStreamDestroy(strm);
+    }
+
+    int init(int alg, int ns, int ups, int spb, int cbw, int cci, int ndo, int dev) override
+    {
+        if (alg != MPGOS_RK4 && alg != MPGOS_RKCK45) { set_error("unknown algorithm"); return MPGOS_E_ARG; }
+        if (ns <= 0 || ups <= 0 || spb <= 0 || ndo < 0 || cbw < 0 || (cci != 0 && cci != 1)) {
+            set_error("per-block: NS, UPS, SPB must be positive, NDO >= 0, CBW >= 0, CCI in {0, 1}");
+            return MPGOS_E_ARG;
+        }
+        if ((long)spb * ups > kMaxBlock) { set_error("per-block: SPB * UPS must not exceed 1024"); return MPGOS_E_ARG; }
+        if (cbw > 0 && 2 * cbw + 1 > ups) { set_error("per-block: 2 * CBW + 1 must not exceed UPS"); return MPGOS_E_ARG; }
+        algorithm = alg; NS = ns; UPS = ups; SPB = spb; CBW = cbw; CCI = cci; NDO = ndo; device = dev; nActive = ns;
+        if (CCI == 0 && CBW == 0) cMatSize = UPS * UPS;
+        else if (CCI == 0) cMatSize = UPS * (2 * CBW + 1);
+        else if (CBW > 0) cMatSize = 2 * CBW + 1;
+        else cMatSize = UPS;
+        MPGOS_HIP_TRY(hipSetDevice(device));
+        MPGOS_HIP_TRY(hipStreamCreateWithFlags(&strm, hipStreamNonBlocking));
+        MPGOS_HIP_TRY(hipEventCreateWithFlags(&syncEvent, hipEventDisableTiming));
+        MPGOS_HIP_TRY(hipEventCreate(&evStart));
+        MPGOS_HIP_TRY(hipEventCreate(&evStop));
+        hipDeviceProp_t prop;
+        MPGOS_HIP_TRY(hipGetDeviceProperties(&prop, device));
+        ldsMax = prop.sharedMemPerBlock;
+        const size_t NU = (size_t)NS * UPS;
+        int rc;
+        if ((rc = X.alloc(UD * NU)) || (rc = uPAR.alloc(NUP * NU)) || (rc = uACC.alloc(NUA * NU)) ||
+            (rc = uACCi.alloc(NiUA * NU)) || (rc = TD.alloc(2 * (size_t)NS)) || (rc = T.alloc(NS)) ||
+            (rc = sPAR.alloc((size_t)NSP * NS)) || (rc = sACC.alloc((size_t)NSA * NS)) || (rc = sACCi.alloc((size_t)NiSA * NS)) ||
+            (rc = strength.alloc((size_t)NC * NS)) || (rc = DOidx.alloc(NS)) || (rc = DOt.alloc((size_t)NDO * NS)) ||
+            (rc = DOx.alloc((size_t)NDO * UD * NU)) || (rc = gPAR.alloc(NGP)) || (rc = gPARi.alloc(NiGP)) ||
+            (rc = cIndex.alloc(NC)) || (rc = cMat.alloc((size_t)NC * cMatSize)) || (rc = status.alloc(NS)) ||
+            (rc = nacc.alloc(NS)) || (rc = nrej.alloc(NS)))
+            return rc;
+        cMatOriginal.assign((size_t)NC * UPS * UPS, P(0));
+        rtol.assign(UD, 1e-8);
+        atol.assign(UD, 1e-8);
+        etol.assign(NE, 1e-6);
+        edir.assign(NE, 0);
+        if ((rc = copy(status, true)) || (rc = copy(nacc, true)) || (rc = copy(nrej, true))) return rc;
+        return sync(MPGOS_PB_ALL, true);
+    }
+
+    int dims(mpgos_pb_dims* d) const override
+    {
+        d->ns = NS; d->ups = UPS; d->ud = UD; d->spb = SPB; d->nc = NC; d->cbw = CBW; d->cci = CCI;
+        d->nup = NUP; d->nsp = NSP; d->ngp = NGP; d->nigp = NiGP; d->nua = NUA; d->niua = NiUA; d->nsa = NSA;
+        d->nisa = NiSA; d->ne = NE; d->ndo = NDO; d->algorithm = algorithm;
+        d->precision = std::is_same<P, float>::value ? MPGOS_FLOAT : MPGOS_DOUBLE;
+        return MPGOS_OK;
+    }
+
+    static int bound(const char* fn, const char* what, int v, int lo, int hi) { return bound_check(fn, what, v, lo, hi); }
+
+    int option(int opt, int idx, double v) override
+    {
+        switch (opt) {
+        case MPGOS_PB_INITIAL_TIME_STEP: dt0 = v; return MPGOS_OK;
+        case MPGOS_PB_ACTIVE_SYSTEMS:
+            if ((int)v < 0 || (int)v > NS) { set_error("ActiveSystems must be in 0..NS"); return MPGOS_E_ARG; }
+            nActive = (int)v; return MPGOS_OK;
+        case MPGOS_PB_MAXIMUM_TIME_STEP: dtmax = v; return MPGOS_OK;
+        case MPGOS_PB_MINIMUM_TIME_STEP: dtmin = v; return MPGOS_OK;
+        case MPGOS_PB_TIME_STEP_GROW_LIMIT: grow = v; return MPGOS_OK;
+        case MPGOS_PB_TIME_STEP_SHRINK_LIMIT: shrink = v; return MPGOS_OK;
+        case MPGOS_PB_RELATIVE_TOLERANCE:
+            if (int rc = bound("SolverOption", "RelativeTolerance", idx, 0, UD - 1)) return rc;
+            rtol[idx] = v; return MPGOS_OK;
+        case MPGOS_PB_ABSOLUTE_TOLERANCE:
+            if (int rc = bound("SolverOption", "AbsoluteTolerance", idx, 0, UD - 1)) return rc;
+            atol[idx] = v; return MPGOS_OK;
+        case MPGOS_PB_EVENT_TOLERANCE:
+            if (int rc = bound("SolverOption", "EventTolerance", idx, 0, NE - 1)) return rc;
+            etol[idx] = v; return MPGOS_OK;
+        case MPGOS_PB_EVENT_DIRECTION:
+            if (int rc = bound("SolverOption", "EventDirection", idx, 0, NE - 1)) return rc;
+            edir[idx] = (int)v; return MPGOS_OK;
+        case MPGOS_PB_DENSE_OUTPUT_SAVE_FREQUENCY: doFreq = (int)v; return MPGOS_OK;
+        case MPGOS_PB_DENSE_OUTPUT_MINIMUM_TIME_STEP: doMinDt = v; return MPGOS_OK;
+        case MPGOS_PB_SHARED_GLOBAL_VARIABLES: sharedGlobals = (int)v != 0; return MPGOS_OK;
+        case MPGOS_PB_SHARED_COUPLING_MATRICES: sharedMatrix = (int)v != 0; return MPGOS_OK;
+        default:
+            set_error("SolverOption: unknown or not applicable option " + std::to_string(opt));
+            return MPGOS_E_ARG;
+        }
+    }
+
+    // Maps (variable, a0..a3) to an element of a host mirror; index rules and
+    // bound checks of PB-IF:868-1140.
+    template <class F>
+    int access(const char* fn, int var, int a0, int a1, int a2, int a3, bool setting, double v, F&& f)
+    {
+        int rc;
+        const size_t NU = (size_t)NS * UPS;
+        switch (var) {
+        case MPGOS_PB_ACTUAL_STATE: case MPGOS_PB_UNIT_PARAMETERS: case MPGOS_PB_UNIT_ACCESSORIES:
+        case MPGOS_PB_INTEGER_UNIT_ACCESSORIES: {
+            if ((rc = bound(fn, "SystemNumber", a0, 0, NS - 1)) || (rc = bound(fn, "UnitNumber", a1, 0, UPS - 1))) return rc;
+            const size_t idx = (size_t)a0 * UPS + a1 + (size_t)a2 * NU;
+            switch (var) {
+            case MPGOS_PB_ACTUAL_STATE: if ((rc = bound(fn, "ActualState", a2, 0, UD - 1))) return rc; f(&X.h[idx]); break;
+            case MPGOS_PB_UNIT_PARAMETERS: if ((rc = bound(fn, "UnitParameters", a2, 0, NUP - 1))) return rc; f(&uPAR.h[idx]); break;
+            case MPGOS_PB_UNIT_ACCESSORIES: if ((rc = bound(fn, "UnitAccessories", a2, 0, NUA - 1))) return rc; f(&uACC.h[idx]); break;
+            default: if ((rc = bound(fn, "IntegerUnitAccessories", a2, 0, NiUA - 1))) return rc; f(&uACCi.h[idx]); break;
+            }
+            return MPGOS_OK;
+        }
+        case MPGOS_PB_TIME_DOMAIN: case MPGOS_PB_SYSTEM_PARAMETERS: case MPGOS_PB_SYSTEM_ACCESSORIES:
+        case MPGOS_PB_INTEGER_SYSTEM_ACCESSORIES: case MPGOS_PB_DENSE_TIME: case MPGOS_PB_COUPLING_STRENGTH: {
+            if ((rc = bound(fn, "SystemNumber", a0, 0, NS - 1))) return rc;
+            const size_t idx = (size_t)a0 + (size_t)a2 * NS;
+            switch (var) {
+            case MPGOS_PB_TIME_DOMAIN: if ((rc = bound(fn, "TimeDomain", a2, 0, 1))) return rc; f(&TD.h[idx]); break;
+            case MPGOS_PB_SYSTEM_PARAMETERS: if ((rc = bound(fn, "SystemParameters", a2, 0, NSP - 1))) return rc; f(&sPAR.h[idx]); break;
+            case MPGOS_PB_SYSTEM_ACCESSORIES: if ((rc = bound(fn, "SystemAccessories", a2, 0, NSA - 1))) return rc; f(&sACC.h[idx]); break;
+            case MPGOS_PB_INTEGER_SYSTEM_ACCESSORIES: if ((rc = bound(fn, "IntegerSystemAccessories", a2, 0, NiSA - 1))) return rc; f(&sACCi.h[idx]); break;
+            case MPGOS_PB_DENSE_TIME: if ((rc = bound(fn, "DenseTime", a2, 0, NDO - 1))) return rc; f(&DOt.h[idx]); break;
+            default: if ((rc = bound(fn, "CouplingStrength", a2, 0, NC - 1))) return rc; f(&strength.h[idx]); break;
+            }
+            return MPGOS_OK;
+        }
+        case MPGOS_PB_GLOBAL_PARAMETERS:
+            if ((rc = bound(fn, "GlobalParameters", a2, 0, NGP - 1))) return rc;
+            f(&gPAR.h[a2]); return MPGOS_OK;
+        case MPGOS_PB_INTEGER_GLOBAL_PARAMETERS:
+            if ((rc = bound(fn, "IntegerGlobalParameters", a2, 0, NiGP - 1))) return rc;
+            f(&gPARi.h[a2]); return MPGOS_OK;
+        case MPGOS_PB_COUPLING_INDEX:
+            if ((rc = bound(fn, "CouplingIndex", a2, 0, NC - 1))) return rc;
+            if (setting && ((int)v < 0 || (int)v >= UD)) { set_error(std::string(fn) + ": CouplingIndex value must be a component 0..UD-1"); return MPGOS_E_ARG; }
+            f(&cIndex.h[a2]); return MPGOS_OK;
+        case MPGOS_PB_COUPLING_MATRIX: {
+            // a0 = coupling, a1 = row, a2 = col (PB-IF:994-1081)
+            if ((rc = bound(fn, "CouplingNumber", a0, 0, NC - 1)) || (rc = bound(fn, "Coupling Matrix Row", a1, 0, UPS - 1)) ||
+                (rc = bound(fn, "Coupling Matrix Col", a2, 0, UPS - 1)))
+                return rc;
+            const size_t orig = (size_t)a1 + (size_t)a2 * UPS + (size_t)a0 * UPS * UPS;
+            if (!setting) { f(&cMatOriginal[orig]); return MPGOS_OK; }
+            size_t idx;
+            if (CCI == 0 && CBW == 0) idx = orig;
+            else {
+                int mc = a2 - a1 + CBW;
+                if (mc >= UPS) mc -= UPS;
+                if (mc < 0) mc += UPS;
+                const int lim = CBW > 0 ? 2 * CBW + 1 : UPS;
+                if ((rc = bound(fn, "Coupling Matrix Shifted Diagonal", mc, 0, lim - 1))) return rc;
+                if (CCI == 0) idx = (size_t)a1 + (size_t)mc * UPS + (size_t)a0 * UPS * lim;
+                else idx = (size_t)mc + (size_t)a0 * lim;
+            }
+            cMatOriginal[orig] = (P)v;
+            f(&cMat.h[idx]);
+            return MPGOS_OK;
+        }
+        case MPGOS_PB_DENSE_INDEX: case MPGOS_PB_ACTUAL_TIME: case MPGOS_PB_ACCEPTED_STEPS: case MPGOS_PB_REJECTED_STEPS:
+        case MPGOS_PB_STATUS:
+            if ((rc = bound(fn, "SystemNumber", a0, 0, NS - 1))) return rc;
+            switch (var) {
+            case MPGOS_PB_DENSE_INDEX:
+                if (setting && (rc = bound(fn, "DenseIndex-Value", (int)v, 0, NDO - 1))) return rc;
+                f(&DOidx.h[a0]); break;
+            case MPGOS_PB_ACTUAL_TIME: f(&T.h[a0]); break;
+            case MPGOS_PB_ACCEPTED_STEPS: f(&nacc.h[a0]); break;
+            case MPGOS_PB_REJECTED_STEPS: f(&nrej.h[a0]); break;
+            default: f(&status.h[a0]); break;
+            }
+            return MPGOS_OK;
+        case MPGOS_PB_DENSE_STATE: {
+            if ((rc = bound(fn, "SystemNumber", a0, 0, NS - 1)) || (rc = bound(fn, "UnitNumber", a1, 0, UPS - 1)) ||
+                (rc = bound(fn, "ComponentNumber in dense state", a2, 0, UD - 1)) || (rc = bound(fn, "DenseState", a3, 0, NDO - 1)))
+                return rc;
+            f(&DOx.h[(size_t)a0 * UPS + a1 + (size_t)a2 * NU + (size_t)a3 * NU * UD]);
+            return MPGOS_OK;
+        }
+        default:
+            set_error(std::string(fn) + ": variable not applicable with this argument configuration");
+            return MPGOS_E_ARG;
+        }
+    }
+
+    int set_host(int var, int a0, int a1, int a2, int a3, double v) override
+    {
+        if (var == MPGOS_PB_ACCEPTED_STEPS || var == MPGOS_PB_REJECTED_STEPS || var == MPGOS_PB_STATUS) {
+            set_error("SetHost: output-only variable");
+            return MPGOS_E_ARG;
+        }
+        return access("SetHost", var, a0, a1, a2, a3, true, v, [&](auto* p) {
+            using T_ = typename std::remove_pointer<decltype(p)>::type;
+            *p = std::is_integral<T_>::value ? (T_)(int)v : (T_)v;
+        });
+    }
+
+    int get_host(int var, int a0, int a1, int a2, int a3, double* v) const override
+    {
+        return const_cast<Solver*>(this)->access("GetHost", var, a0, a1, a2, a3, false, 0.0, [&](auto* p) { *v = (double)*p; });
+    }
+
+    template <class T_> static void* ret(Buffer<T_>& b, bool host, size_t* nbytes)
+    {
+        if (nbytes) *nbytes = b.bytes();
+        return host ? (void*)b.h : (void*)b.d;
+    }
+
+    void* buffer(int var, bool host, size_t* nbytes) override
+    {
+        switch (var) {
+        case MPGOS_PB_TIME_DOMAIN: return ret(TD, host, nbytes);
+        case MPGOS_PB_ACTUAL_TIME: return ret(T, host, nbytes);
+        case MPGOS_PB_ACTUAL_STATE: return ret(X, host, nbytes);
+        case MPGOS_PB_UNIT_PARAMETERS: return ret(uPAR, host, nbytes);
+        case MPGOS_PB_SYSTEM_PARAMETERS: return ret(sPAR, host, nbytes);
+        case MPGOS_PB_GLOBAL_PARAMETERS: return ret(gPAR, host, nbytes);
+        case MPGOS_PB_INTEGER_GLOBAL_PARAMETERS: return ret(gPARi, host, nbytes);
+        case MPGOS_PB_UNIT_ACCESSORIES: return ret(uACC, host, nbytes);
+        case MPGOS_PB_INTEGER_UNIT_ACCESSORIES: return ret(uACCi, host, nbytes);
+        case MPGOS_PB_SYSTEM_ACCESSORIES: return ret(sACC, host, nbytes);
+        case MPGOS_PB_INTEGER_SYSTEM_ACCESSORIES: return ret(sACCi, host, nbytes);
+        case MPGOS_PB_COUPLING_MATRIX: return ret(cMat, host, nbytes);
+        case MPGOS_PB_COUPLING_STRENGTH: return ret(strength, host, nbytes);
+        case MPGOS_PB_COUPLING_INDEX: return ret(cIndex, host, nbytes);
+        case MPGOS_PB_DENSE_INDEX: return ret(DOidx, host, nbytes);
+        case MPGOS_PB_DENSE_TIME: return ret(DOt, host, nbytes);
+        case MPGOS_PB_DENSE_STATE: return ret(DOx, host, nbytes);
+        case MPGOS_PB_ACCEPTED_STEPS: return ret(nacc, host, nbytes);
+        case MPGOS_PB_REJECTED_STEPS: return ret(nrej, host, nbytes);
+        case MPGOS_PB_STATUS: return ret(status, host, nbytes);
+        default:
+            set_error("buffer: variable has no single buffer");
+            if (nbytes) *nbytes = 0;
+            return nullptr;
+        }
+    }
+
+    template <class T_> int copy(Buffer<T_>& b, bool to_device)
+    {
+        if (b.n == 0) return MPGOS_OK;
+        if (to_device) MPGOS_HIP_TRY(hipMemcpyAsync(b.d, b.h, b.bytes(), hipMemcpyHostToDevice, strm));
+        else MPGOS_HIP_TRY(hipMemcpyAsync(b.h, b.d, b.bytes(), hipMemcpyDeviceToHost, strm));
+        return MPGOS_OK;
+    }
+
+    // PB-IF:1143-1337
+    int sync(int var, bool to_device) override
+    {
+        MPGOS_HIP_TRY(hipSetDevice(device));
+        int rc = MPGOS_OK;
+        switch (var) {
+        case MPGOS_PB_ALL:
+            for (int v = MPGOS_PB_TIME_DOMAIN; v <= MPGOS_PB_DENSE_STATE; v++) {
+                if (v == MPGOS_PB_DENSE_OUTPUT) continue;
+                if ((rc = sync(v, to_device))) return rc;
+            }
+            if (!to_device && ((rc = copy(nacc, false)) || (rc = copy(nrej, false)) || (rc = copy(status, false)))) return rc;
+            return MPGOS_OK;
+        case MPGOS_PB_DENSE_OUTPUT:
+            if ((rc = copy(DOidx, to_device)) || (rc = copy(DOt, to_device)) || (rc = copy(DOx, to_device))) return rc;
+            return MPGOS_OK;
+        case MPGOS_PB_ACCEPTED_STEPS: case MPGOS_PB_REJECTED_STEPS: case MPGOS_PB_STATUS:
+            if (to_device) { set_error("SynchroniseFromHostToDevice: output-only variable"); return MPGOS_E_ARG; }
+            return var == MPGOS_PB_ACCEPTED_STEPS ? copy(nacc, false) : var == MPGOS_PB_REJECTED_STEPS ? copy(nrej, false) : copy(status, false);
+        default: {
+            switch (var) {
+            case MPGOS_PB_TIME_DOMAIN: return copy(TD, to_device);
+            case MPGOS_PB_ACTUAL_TIME: return copy(T, to_device);
+            case MPGOS_PB_ACTUAL_STATE: return copy(X, to_device);
+            case MPGOS_PB_UNIT_PARAMETERS: return copy(uPAR, to_device);
+            case MPGOS_PB_SYSTEM_PARAMETERS: return copy(sPAR, to_device);
+            case MPGOS_PB_GLOBAL_PARAMETERS: return copy(gPAR, to_device);
+            case MPGOS_PB_INTEGER_GLOBAL_PARAMETERS: return copy(gPARi, to_device);
+            case MPGOS_PB_UNIT_ACCESSORIES: return copy(uACC, to_device);
+            case MPGOS_PB_INTEGER_UNIT_ACCESSORIES: return copy(uACCi, to_device);
+            case MPGOS_PB_SYSTEM_ACCESSORIES: return copy(sACC, to_device);
+            case MPGOS_PB_INTEGER_SYSTEM_ACCESSORIES: return copy(sACCi, to_device);
+            case MPGOS_PB_COUPLING_MATRIX: return copy(cMat, to_device);
+            case MPGOS_PB_COUPLING_STRENGTH: return copy(strength, to_device);
+            case MPGOS_PB_COUPLING_INDEX: return copy(cIndex, to_device);
+            case MPGOS_PB_DENSE_INDEX: return copy(DOidx, to_device);
+            case MPGOS_PB_DENSE_TIME: return copy(DOt, to_device);
+            case MPGOS_PB_DENSE_STATE: return copy(DOx, to_device);
+            default: set_error("Synchronise: variable not applicable"); return MPGOS_E_ARG;
+            }
+        }
+        }
+    }
+
+    Args<Sys, P> make_args() const
+    {
+        Args<Sys, P> a;
+        std::memset(&a, 0, sizeof a);
+        a.X = X.d; a.uPAR = uPAR.d; a.uACC = uACC.d; a.uACCi = uACCi.d;
+        a.TD = TD.d; a.T = T.d; a.sPAR = sPAR.d; a.sACC = sACC.d; a.sACCi = sACCi.d; a.strength = strength.d;
+        a.DOidx = DOidx.d; a.DOt = DOt.d; a.DOx = DOx.d;
+        a.gPAR = gPAR.d; a.gPARi = gPARi.d; a.cIndex = cIndex.d; a.cMat = cMat.d;
+        a.nacc = nacc.d; a.nrej = nrej.d; a.status = status.d;
+        a.NS = NS; a.UPS = UPS; a.SPB = SPB; a.nActive = nActive; a.NDO = NDO; a.CBW = CBW; a.CCI = CCI;
+        a.cMatSize = cMatSize; a.sharedGlobals = sharedGlobals; a.doFreq = doFreq;
+        a.dt0 = (P)dt0; a.dtmax = (P)dtmax; a.dtmin = (P)dtmin; a.grow = (P)grow; a.shrink = (P)shrink; a.doMinDt = (P)doMinDt;
+        for (int i = 0; i < UD; i++) { a.rtol[i] = (P)rtol[i]; a.atol[i] = (P)atol[i]; }
+        for (int i = 0; i < NE; i++) { a.etol[i] = (P)etol[i]; a.edir[i] = edir[i]; }
+        return a;
+    }
+
+    size_t lds_bytes(int sm) const
+    {
+        size_t o1, o2, o3, o4;
+        return lds_layout<Sys, P>(SPB, UPS, sharedGlobals, sm, cMatSize, &o1, &o2, &o3, &o4);
+    }
+
+    // PB-IF:1949-1965
+    int solve() override
+    {
+        MPGOS_HIP_TRY(hipSetDevice(device));
+        Args<Sys, P> a = make_args();
+        // the coupling matrix goes to LDS only when asked for and when it fits
+        a.sharedMatrix = sharedMatrix && lds_bytes(1) <= ldsMax ? 1 : 0;
+        const size_t lds = lds_bytes(a.sharedMatrix);
+        if (lds > ldsMax) { set_error("per-block: system scope and coupling terms exceed the LDS of a workgroup"); return MPGOS_E_ARG; }
+        const int threads = ((SPB * UPS + 63) / 64) * 64;
+        const int grid = NS / SPB + (NS % SPB == 0 ? 0 : 1);
+        MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+        if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4>), dim3(grid), dim3(threads), lds, strm, a);
+        else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45>), dim3(grid), dim3(threads), lds, strm, a);
+        MPGOS_HIP_TRY(hipGetLastError());
+        MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
+        return MPGOS_OK;
+    }
+
+    int synchronise_device() override { MPGOS_HIP_TRY(hipSetDevice(device)); MPGOS_HIP_TRY(hipDeviceSynchronize()); return MPGOS_OK; }
+    int insert_sync_point() override { MPGOS_HIP_TRY(hipSetDevice(device)); MPGOS_HIP_TRY(hipEventRecord(syncEvent, strm)); return MPGOS_OK; }
+    int synchronise_solver() override { MPGOS_HIP_TRY(hipSetDevice(device)); MPGOS_HIP_TRY(hipEventSynchronize(syncEvent)); return MPGOS_OK; }
+    int last_solve_ms(double* ms) const override
+    {
+        float f = 0.f;
+        MPGOS_HIP_TRY(hipEventElapsedTime(&f, evStart, evStop));
+        *ms = f;
+        return MPGOS_OK;
+    }
+    void* stream() override { return (void*)strm; }
+};
+
+} // namespace pb
+} // namespace mpgos

@@ -34,6 +34,20 @@ void set_error(const std::string& msg);   // thread-local message for mpgos_last
         }                                                                                \
     } while (0)
 
+// Index bound check with the reference's message (PT-IF:565-581), returning
+// MPGOS_E_ARG instead of exiting.
+inline int bound_check(const char* fn, const char* what, int v, int lo, int hi)
+{
+    if (v < lo || v > hi) {
+        char buf[256];
+        if (lo > hi) std::snprintf(buf, sizeof buf, "%s: %s index %d out of range (acceptable index: none)", fn, what, v);
+        else std::snprintf(buf, sizeof buf, "%s: %s index %d out of range (acceptable index: %d-%d)", fn, what, v, lo, hi);
+        set_error(buf);
+        return MPGOS_E_ARG;
+    }
+    return MPGOS_OK;
+}
+
 namespace pt {
 
 // Kernel entry points.  WPE is the waves-per-EU hint of the register-budget
@@ -200,17 +214,7 @@ public:
         return MPGOS_OK;
     }
 
-    static int bound(const char* fn, const char* what, int v, int lo, int hi)
-    {
-        if (v < lo || v > hi) {
-            char buf[256];
-            if (lo > hi) std::snprintf(buf, sizeof buf, "%s: %s index %d out of range (acceptable index: none)", fn, what, v);
-            else std::snprintf(buf, sizeof buf, "%s: %s index %d out of range (acceptable index: %d-%d)", fn, what, v, lo, hi);
-            set_error(buf);
-            return MPGOS_E_ARG;
-        }
-        return MPGOS_OK;
-    }
+    static int bound(const char* fn, const char* what, int v, int lo, int hi) { return bound_check(fn, what, v, lo, hi); }
 
     int option(int opt, int idx, double v) override
     {

@@ -3,6 +3,7 @@
 // parallelises; the C ABI (capi.hip) selects a factory by name.
 #pragma once
 
+#include "pb_engine.h"
 #include "pt_engine.h"
 
 #define MPGOS_PT_FACTORY(NAME, SYS)                                                   \
@@ -16,7 +17,27 @@
         }                                                                             \
     }
 
+#define MPGOS_PB_FACTORY(NAME, SYS)                                                   \
+    ::mpgos::pb::SolverBase* make_pb_##NAME(int precision)                            \
+    {                                                                                 \
+        switch (precision) {                                                          \
+        case MPGOS_DOUBLE: return new ::mpgos::pb::Solver<SYS, double>();             \
+        case MPGOS_FLOAT: return new ::mpgos::pb::Solver<SYS, float>();               \
+        default: return nullptr;                                                      \
+        }                                                                             \
+    }
+
 namespace mpgos {
+namespace pb {
+struct Registration {
+    const char* name;
+    SolverBase* (*make)(int precision);
+};
+SolverBase* make_pb_ring_duffing(int precision);
+SolverBase* make_pb_ring_dw(int precision);
+SolverBase* make_pb_pb_linear(int precision);
+} // namespace pb
+
 namespace pt {
 
 struct Registration {

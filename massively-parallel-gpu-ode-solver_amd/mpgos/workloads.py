@@ -231,3 +231,167 @@ def read(solver):
         out["evlog"] = solver.host_array(V.EventLog).copy()
         out["evlogidx"] = solver.host_array(V.EventLogIndex).copy()
     return out
+
+
+# ---------------------------------------------------------------- per-block specs
+PB_SYSTEM_DIMS = {
+    #               ud nc nup nsp ngp nigp nua niua nsa nisa ne
+    "ring_duffing": (2, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0),
+    "ring_dw":      (2, 2, 3, 0, 1, 0, 3, 1, 2, 1, 1),
+    "pb_linear":    (1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0),
+}
+
+
+def coupling_storage_size(ups, cbw, cci):
+    if cci == 0 and cbw == 0:
+        return ups * ups
+    if cci == 0:
+        return ups * (2 * cbw + 1)
+    return 2 * cbw + 1 if cbw > 0 else ups
+
+
+def store_coupling(dense, cbw, cci):
+    """A full UPS x UPS matrix in the reference's storage form (PB-IF:994-1081):
+    returns the stored vector (entries outside the band must be zero, and a
+    circular form must be constant along its diagonals)."""
+    ups = dense.shape[0]
+    if cci == 0 and cbw == 0:
+        return np.asarray(dense, np.float64).T.ravel().copy()          # M[row + col*UPS]
+    W = 2 * cbw + 1 if cbw > 0 else ups
+    out = np.zeros(ups * W if cci == 0 else W)
+    for r in range(ups):
+        for c in range(ups):
+            mc = (c - r + cbw) % ups
+            if mc >= W:
+                continue
+            if cci == 0:
+                out[r + mc * ups] = dense[r, c]
+            else:
+                out[mc] = dense[r, c]
+    return out
+
+
+def new_pb_spec(system, algorithm, ns, ups, cbw=0, cci=0, ndo=0):
+    ud, nc, nup, nsp, ngp, nigp, nua, niua, nsa, nisa, ne = PB_SYSTEM_DIMS[system]
+    nu = ns * ups
+    return {
+        "system": system, "algorithm": int(algorithm), "ns": ns, "ups": ups, "cbw": cbw, "cci": cci, "ndo": ndo,
+        "ud": ud, "nc": nc, "nup": nup, "nsp": nsp, "ngp": ngp, "nigp": nigp, "nua": nua, "niua": niua, "nsa": nsa,
+        "nisa": nisa, "ne": ne, "options": {}, "rtol": 1e-8, "atol": 1e-8, "etol": 1e-6, "edir": 0,
+        "X": np.zeros((ud, nu)), "uPAR": np.zeros((nup, nu)), "uACC": np.zeros((nua, nu)),
+        "uACCi": np.zeros((niua, nu), np.int32), "TD": np.zeros((2, ns)), "T": np.zeros(ns),
+        "sPAR": np.zeros((nsp, ns)), "sACC": np.zeros((nsa, ns)), "sACCi": np.zeros((nisa, ns), np.int32),
+        "strength": np.zeros((nc, ns)), "DOidx": np.zeros(ns, np.int32), "gPAR": np.zeros(ngp),
+        "gPARi": np.zeros(nigp, np.int32), "cIndex": np.zeros(nc, np.int32),
+        "cMat": np.zeros(nc * coupling_storage_size(ups, cbw, cci)),
+    }
+
+
+def config4_ring(ns=4096, ups=256, periods=4, algorithm=Algorithms.RKCK45):
+    """Per-block ring of U forced Duffing units with a dense coupling matrix on
+    the velocity equation: C_uv ~ N(0, 0.05/U) (variance), zero diagonal, one
+    matrix shared by all systems; per-unit k = 0.2, per-system B ~ U[0.5, 5];
+    unit initial states (x, v) ~ U[-1, 1]^2; RKCK45 at rtol = atol = 1e-9 over
+    `periods` forcing periods (4 by default)."""
+    s = new_pb_spec("ring_duffing", algorithm, ns, ups)
+    rs = Stream(SEED, 4)
+    C = rs.normal(ups * ups, np.sqrt(0.05 / ups)).reshape(ups, ups)
+    np.fill_diagonal(C, 0.0)
+    s["dense_coupling"] = C
+    s["cMat"] = store_coupling(C, 0, 0)
+    s["uPAR"][0] = 0.2
+    s["sPAR"][0] = rs.uniform(ns, 0.5, 5.0)
+    s["X"][:] = rs.uniform(2 * ns * ups, -1.0, 1.0).reshape(2, ns * ups)
+    s["TD"][1] = periods * (2.0 * PI)
+    s["strength"][0] = 1.0
+    s["cIndex"][0] = 1
+    s["rtol"] = 1e-9
+    s["atol"] = 1e-9
+    return s
+
+
+_PB_OPT = {"dt0": 0, "dtmax": 2, "dtmin": 3, "grow": 4, "shrink": 5, "do_freq": 10, "do_min_dt": 11,
+           "shared_globals": 12, "shared_matrix": 13}
+
+
+def load_pb(solver, spec):
+    """Options and inputs of a per-block spec into a PBProblemSolver, then H2D."""
+    from .perblock import PB
+    for k, v in spec["options"].items():
+        solver.SolverOption(_PB_OPT[k], v)
+    if "n_active" in spec:
+        solver.SolverOption(1, spec["n_active"])
+    ud, ne = spec["ud"], spec["ne"]
+    rtol = np.broadcast_to(np.asarray(spec["rtol"], np.float64), (ud,))
+    atol = np.broadcast_to(np.asarray(spec["atol"], np.float64), (ud,))
+    for i in range(ud):
+        solver.SolverOption(6, i, rtol[i])
+        solver.SolverOption(7, i, atol[i])
+    if ne:
+        etol = np.broadcast_to(np.asarray(spec["etol"], np.float64), (ne,))
+        edir = np.broadcast_to(np.asarray(spec["edir"], np.int64), (ne,))
+        for i in range(ne):
+            solver.SolverOption(8, i, etol[i])
+            solver.SolverOption(9, i, int(edir[i]))
+    fill = {PB.ActualState: "X", PB.UnitParameters: "uPAR", PB.UnitAccessories: "uACC",
+            PB.IntegerUnitAccessories: "uACCi", PB.TimeDomain: "TD", PB.ActualTime: "T",
+            PB.SystemParameters: "sPAR", PB.SystemAccessories: "sACC", PB.IntegerSystemAccessories: "sACCi",
+            PB.CouplingStrength: "strength", PB.DenseIndex: "DOidx", PB.GlobalParameters: "gPAR",
+            PB.IntegerGlobalParameters: "gPARi", PB.CouplingIndex: "cIndex", PB.CouplingMatrix: "cMat"}
+    for var, key in fill.items():
+        dst = solver.host_array(var)
+        if dst.size:
+            dst[...] = np.asarray(spec[key]).reshape(dst.shape)
+    solver.SynchroniseFromHostToDevice(PB.All)
+
+
+def read_pb(solver):
+    from .perblock import PB
+    out = {}
+    for var, key in {PB.ActualState: "X", PB.UnitAccessories: "uACC", PB.IntegerUnitAccessories: "uACCi",
+                     PB.TimeDomain: "TD", PB.ActualTime: "T", PB.SystemAccessories: "sACC",
+                     PB.IntegerSystemAccessories: "sACCi", PB.DenseIndex: "DOidx", PB.AcceptedSteps: "nacc",
+                     PB.RejectedSteps: "nrej", PB.Status: "status"}.items():
+        out[key] = solver.host_array(var).copy()
+    if solver.d.ndo:
+        out["DOt"] = solver.host_array(PB.DenseTime).copy()
+        out["DOx"] = solver.host_array(PB.DenseState).copy()
+    return out
+
+
+def tutorial_ring_dw(algorithm=Algorithms.RKCK45, ns=25, ups=14):
+    """The reference's per-block tutorial (T1_RingOfDuffingOscillators/
+    RingOfDuffingOscillators.cu:17-270): 25 systems of 14 double-well Duffing
+    units on a circular nearest-neighbour ring (CBW 1, CCI 1, two couplings
+    of strength 0.004/2), unit 11 started from a 5 x 5 grid of initial
+    conditions and every other unit from (-5, -5); uPAR = (0.5, 0.24, 13.633),
+    gPAR = 0.004, horizon 10 periods of 2pi/0.5, dt0 = 1e-2, tolerances
+    1e-8, event x' = 0 in direction -1 (local maxima), dense output of 500
+    points with minimum spacing 1e-6."""
+    s = new_pb_spec("ring_dw", algorithm, ns, ups, cbw=1, cci=1, ndo=500)
+    nx = ny = int(round(np.sqrt(ns)))
+    icx = linspace(-8.0, 8.0, nx)
+    icy = linspace(-20.0, 40.0, ny)
+    cps = 0.004 / 2.0
+    for sid in range(ns):
+        x0, y0 = icx[sid // ny], icy[sid % ny]
+        s["TD"][1, sid] = 10.0 * 2.0 * PI / 0.5
+        s["strength"][:, sid] = cps
+        s["sACC"][0, sid] = 1e-2
+        s["sACC"][1, sid] = 1e-2
+        for u in range(ups):
+            g = sid * ups + u
+            a, b = (x0, y0) if u == 11 else (-5.0, -5.0)
+            s["X"][:, g] = (a, b)
+            s["uACC"][0, g] = a
+            s["uACC"][1, g] = b
+            s["uPAR"][:, g] = (0.5, 0.24, 13.633)
+    s["cIndex"][:] = (0, 1)
+    s["gPAR"][0] = 0.004
+    s["cMat"][:] = np.tile([1.0, 0.0, 1.0], 2)
+    s["options"].update(dt0=1e-2, do_min_dt=1e-6, do_freq=1, shared_globals=0)
+    s["rtol"] = 1e-8
+    s["atol"] = 1e-8
+    s["edir"] = -1
+    s["spb"] = 3
+    return s

@@ -9,11 +9,13 @@
 
 #include "dual.h"
 #include "flops.h"
+#include "pb_oracle.h"
 #include "pt_oracle.h"
 #include "systems/affine2.h"
 #include "systems/duffing.h"
 #include "systems/duffing_dw.h"
 #include "systems/lorenz.h"
+#include "systems/pb_ring.h"
 
 namespace oracle {
 namespace flops {
@@ -108,6 +110,21 @@ static const Entry* find(const char* name)
 
 } // namespace oracle
 
+namespace oracle {
+// Per-block coupled systems (pb_oracle.h), OpenMP over systems.
+template <class Sys>
+static void run_pb(const oracle::PBJob& j)
+{
+    const int threads = j.nthreads > 0 ? j.nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+    for (int s = 0; s < j.n_active; s++) {
+        oracle::PBSystem<Sys> sys(j, s);
+        sys.run();
+    }
+}
+
+} // namespace oracle
+
 extern "C" {
 
 int oracle_pt_solve(const char* system, oracle::Job* job, double* seconds)
@@ -135,6 +152,21 @@ int oracle_pt_count_flops(const char* system, oracle::Job* job, long long* flops
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+int oracle_pb_solve(const char* system, oracle::PBJob* job, double* seconds)
+{
+    if (!system || !job) return -4;
+    void (*fn)(const oracle::PBJob&) = nullptr;
+    if (std::strcmp(system, "ring_duffing") == 0) fn = oracle::run_pb<mpgos::systems::RingDuffing>;
+    else if (std::strcmp(system, "ring_dw") == 0) fn = oracle::run_pb<mpgos::systems::RingDW>;
+    else if (std::strcmp(system, "pb_linear") == 0) fn = oracle::run_pb<mpgos::systems::PBLinear>;
+    if (!fn) return -4;
+    const auto t0 = std::chrono::steady_clock::now();
+    fn(*job);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return 0;
+}
 
 int oracle_pt_sensitivity(const char* system, oracle::Job* job, double* jac)
 {

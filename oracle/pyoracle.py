@@ -178,3 +178,71 @@ def sensitivity(system, spec, nthreads=0):
     out = run(system, spec, _job_hook=hook)
     out["jac"] = jac
     return out
+
+
+# ---------------------------------------------------------------- per-block
+class PBJob(ctypes.Structure):
+    _fields_ = [
+        ("ns", ctypes.c_int), ("ups", ctypes.c_int), ("n_active", ctypes.c_int), ("ndo", ctypes.c_int),
+        ("algorithm", ctypes.c_int), ("cbw", ctypes.c_int), ("cci", ctypes.c_int), ("cmat_size", ctypes.c_int),
+        ("dt0", ctypes.c_double), ("dtmax", ctypes.c_double), ("dtmin", ctypes.c_double),
+        ("grow", ctypes.c_double), ("shrink", ctypes.c_double), ("do_min_dt", ctypes.c_double),
+        ("do_freq", ctypes.c_int),
+        ("rtol", _dp), ("atol", _dp), ("etol", _dp), ("edir", _ip),
+        ("X", _dp), ("uPAR", _dp), ("uACC", _dp), ("uACCi", _ip),
+        ("TD", _dp), ("T", _dp), ("sPAR", _dp), ("sACC", _dp), ("sACCi", _ip), ("strength", _dp), ("DOidx", _ip),
+        ("DOt", _dp), ("DOx", _dp),
+        ("gPAR", _dp), ("gPARi", _ip), ("cIndex", _ip), ("cMat", _dp),
+        ("nacc", _lp), ("nrej", _lp), ("status", _ip),
+        ("nthreads", ctypes.c_int),
+    ]
+
+
+def run_pb(system, spec, nthreads=0):
+    """Integrate a per-block spec (see mpgos.workloads.new_pb_spec) with the
+    oracle; returns the output arrays (copies) plus 'seconds'."""
+    L = lib()
+    if not hasattr(L, "_pb_set"):
+        L.oracle_pb_solve.restype = ctypes.c_int
+        L.oracle_pb_solve.argtypes = [ctypes.c_char_p, ctypes.POINTER(PBJob), ctypes.POINTER(ctypes.c_double)]
+        L._pb_set = True
+    ns, ups, ud = spec["ns"], spec["ups"], spec["ud"]
+    nu = ns * ups
+    ndo = spec.get("ndo", 0)
+    o = dict(dt0=1e-2, dtmax=1e6, dtmin=1e-12, grow=5.0, shrink=0.1, do_min_dt=0.0, do_freq=1)
+    o.update(spec.get("options", {}))
+
+    def cp(name, dtype):
+        return np.ascontiguousarray(np.array(spec[name], dtype=dtype, copy=True))
+
+    out = {k: cp(k, np.float64) for k in ("X", "uPAR", "uACC", "TD", "T", "sPAR", "sACC", "strength", "gPAR", "cMat")}
+    out.update({k: cp(k, np.int32) for k in ("uACCi", "sACCi", "gPARi", "cIndex", "DOidx")})
+    out["DOt"] = np.zeros((ndo, ns))
+    out["DOx"] = np.zeros((ndo, ud, nu))
+    out["nacc"] = np.zeros(ns, np.int64)
+    out["nrej"] = np.zeros(ns, np.int64)
+    out["status"] = np.zeros(ns, np.int32)
+    ne = spec["ne"]
+    rtol = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("rtol", 1e-8), np.float64), (ud,)))
+    atol = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("atol", 1e-8), np.float64), (ud,)))
+    etol = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("etol", 1e-6), np.float64), (max(ne, 1),)))
+    edir = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("edir", 0), np.int32), (max(ne, 1),)))
+    j = PBJob()
+    j.ns, j.ups, j.n_active, j.ndo = ns, ups, int(spec.get("n_active", ns)), ndo
+    j.algorithm, j.cbw, j.cci = int(spec["algorithm"]), int(spec["cbw"]), int(spec["cci"])
+    j.cmat_size = out["cMat"].size // max(1, spec["nc"])
+    for k in ("dt0", "dtmax", "dtmin", "grow", "shrink", "do_min_dt"):
+        setattr(j, k, float(o[k]))
+    j.do_freq = int(o["do_freq"])
+    j.rtol, j.atol, j.etol, j.edir = _ptr(rtol, _dp), _ptr(atol, _dp), _ptr(etol, _dp), _ptr(edir, _ip)
+    for k in ("X", "uPAR", "uACC", "TD", "T", "sPAR", "sACC", "strength", "DOt", "DOx", "gPAR", "cMat"):
+        setattr(j, k, _ptr(out[k], _dp))
+    for k in ("uACCi", "sACCi", "DOidx", "gPARi", "cIndex", "status"):
+        setattr(j, k, _ptr(out[k], _ip))
+    j.nacc, j.nrej = _ptr(out["nacc"], _lp), _ptr(out["nrej"], _lp)
+    j.nthreads = int(nthreads)
+    sec = ctypes.c_double()
+    if L.oracle_pb_solve(system.encode(), ctypes.byref(j), ctypes.byref(sec)) != 0:
+        raise RuntimeError(f"oracle per-block failed for {system!r}")
+    out["seconds"] = sec.value
+    return out

@@ -72,3 +72,37 @@ def test_create_without_device_fails_cleanly():
     h = ctypes.c_void_p()
     rc = L.mpgos_pt_create(b"lorenz", 1, 0, 16, 0, 0, ctypes.byref(h))
     assert rc == -2 and not h.value
+
+
+def test_per_block_registry_and_errors():
+    assert set(mpgos.list_pb_systems()) >= {"ring_duffing", "ring_dw", "pb_linear"}
+    L = mpgos.lib()
+    h = ctypes.c_void_p()
+    rc = L.mpgos_pb_create(b"no_such_system", 1, 0, 4, 8, 1, 0, 0, 0, 0, ctypes.byref(h))
+    assert rc == -4 and b"unknown system" in L.mpgos_last_error()
+    assert L.mpgos_pb_solve(None) == -1
+    assert L.mpgos_pb_destroy(None) == -1
+
+
+def test_per_block_enumerations_follow_the_reference_order():
+    # CoupledSystems_PerBlock_Interface.cuh:35-68
+    from mpgos import PB, PBOption
+    ref_vars = ["All", "TimeDomain", "ActualTime", "ActualState", "UnitParameters", "SystemParameters",
+                "GlobalParameters", "IntegerGlobalParameters", "UnitAccessories", "IntegerUnitAccessories",
+                "SystemAccessories", "IntegerSystemAccessories", "CouplingMatrix", "CouplingStrength",
+                "CouplingIndex", "DenseOutput", "DenseIndex", "DenseTime", "DenseState"]
+    ref_opts = ["InitialTimeStep", "ActiveSystems", "MaximumTimeStep", "MinimumTimeStep", "TimeStepGrowLimit",
+                "TimeStepShrinkLimit", "RelativeTolerance", "AbsoluteTolerance", "EventTolerance",
+                "EventDirection", "DenseOutputSaveFrequency", "DenseOutputMinimumTimeStep",
+                "SharedGlobalVariables", "SharedCouplingMatrices"]
+    assert [PB(i).name for i in range(len(ref_vars))] == ref_vars
+    assert [PBOption(i).name for i in range(len(ref_opts))] == ref_opts
+
+
+def test_per_block_create_without_device_fails_cleanly():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    L = mpgos.lib()
+    h = ctypes.c_void_p()
+    rc = L.mpgos_pb_create(b"ring_duffing", 1, 0, 4, 8, 1, 0, 0, 0, 0, ctypes.byref(h))
+    assert rc == -2 and not h.value

@@ -1,0 +1,142 @@
+"""Per-block coupled systems on the GPU, bit-for-bit against the oracle:
+the reference tutorial's double-well ring (events, accessories, dense output,
+several systems per block), linear rings in every coupling storage form with
+the matrix in L2 or staged in LDS, partial activity, and the dense forced
+Duffing ring of 256 units.  Calls go through libmpgos's per-block C ABI."""
+import numpy as np
+import pytest
+
+from mpgos import workloads as W
+from mpgos.enums import Algorithms
+from mpgos.perblock import PB, PBOption, PBProblemSolver
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("X", "T", "TD", "uACC", "uACCi", "sACC", "sACCi", "DOidx", "nacc", "nrej", "status", "DOt", "DOx")
+
+
+def run_pb_gpu(spec, spb=None, shared_matrix=None, shared_globals=None, solves=1):
+    s = PBProblemSolver(spec["system"], spec["ns"], spec["ups"], spec["algorithm"], spb=spb or spec.get("spb", 1),
+                        cbw=spec["cbw"], cci=spec["cci"], ndo=spec["ndo"])
+    try:
+        W.load_pb(s, spec)
+        if shared_matrix is not None:
+            s.SolverOption(PBOption.SharedCouplingMatrices, shared_matrix)
+        if shared_globals is not None:
+            s.SolverOption(PBOption.SharedGlobalVariables, shared_globals)
+        outs = []
+        for _ in range(solves):
+            s.solve_blocking()
+            outs.append(W.read_pb(s))
+        return outs if solves > 1 else outs[0]
+    finally:
+        s.close()
+
+
+def _mask_dense(out):
+    out = dict(out)
+    if "DOt" in out and out["DOt"].size:
+        ndo, ns = out["DOt"].shape
+        valid = np.arange(ndo)[:, None] < np.minimum(out["DOidx"], ndo)[None, :]
+        out["DOt"] = np.where(valid, out["DOt"], 0.0)
+        ups = out["DOx"].shape[2] // ns
+        vu = np.repeat(valid, ups, axis=1)
+        out["DOx"] = np.where(vu[:, None, :], out["DOx"], 0.0)
+    return out
+
+
+def assert_pb_bitwise(gpu, ref, label=""):
+    gpu, ref = _mask_dense(gpu), _mask_dense(ref)
+    for k in KEYS:
+        if k not in gpu or k not in ref:
+            continue
+        a, b = np.asarray(gpu[k]), np.asarray(ref[k])
+        assert a.shape == b.shape, f"{label} {k}: shape {a.shape} vs {b.shape}"
+        same = (a.view(np.uint64) == b.view(np.uint64)) | (np.isnan(a) & np.isnan(b)) if a.dtype == np.float64 else a == b
+        if not same.all():
+            bad = np.argwhere(~same)
+            i = tuple(bad[0])
+            raise AssertionError(f"{label} {k}: {len(bad)} of {a.size} differ; first at {i}: gpu={a[i]!r} oracle={b[i]!r}")
+
+
+@pytest.mark.parametrize("alg", [Algorithms.RKCK45, Algorithms.RK4], ids=["rkck45", "rk4"])
+def test_tutorial_ring_dw(oracle, alg):
+    import pyoracle
+    spec = W.tutorial_ring_dw(algorithm=alg)
+    if alg == Algorithms.RK4:
+        spec["options"]["dt0"] = 1e-3
+    print(f"tutorial ring_dw alg={int(alg)}", flush=True)
+    ref = pyoracle.run_pb("ring_dw", spec)
+    for spb in (3, 1, 4):
+        got = run_pb_gpu(spec, spb=spb)
+        assert_pb_bitwise(got, ref, f"spb={spb}")
+    assert (ref["uACCi"].reshape(-1) <= 2).all() and (ref["nacc"] > 10).all()
+
+
+FORMS = [(0, 0), (0, 2), (1, 2), (1, 0)]
+
+
+@pytest.mark.parametrize("form", FORMS, ids=["full", "banded", "circular_banded", "full_circular"])
+@pytest.mark.parametrize("alg", [Algorithms.RKCK45, Algorithms.RK4], ids=["rkck45", "rk4"])
+def test_linear_ring_forms(oracle, form, alg):
+    import pyoracle
+    cci, cbw = form
+    ns, ups = 37, 24
+    rs = W.Stream(W.SEED, 200 + 2 * cci + cbw)
+    s = W.new_pb_spec("pb_linear", alg, ns, ups, cbw=cbw, cci=cci, ndo=8)
+    Wd = 2 * cbw + 1 if cbw > 0 else ups
+    s["cMat"] = rs.normal(W.coupling_storage_size(ups, cbw, cci)) * (0.5 / np.sqrt(Wd))
+    s["uPAR"][0] = rs.uniform(ns * ups, -1.0, 0.3)
+    s["X"][0] = rs.uniform(ns * ups, -1.0, 1.0)
+    s["strength"][0] = rs.uniform(ns, 0.5, 1.5)
+    s["TD"][1] = rs.uniform(ns, 1.0, 3.0)
+    s["rtol"] = s["atol"] = 1e-9
+    s["options"].update(dt0=1.0 / 128.0, do_freq=3)
+    s["n_active"] = ns - 5
+    print(f"linear ring form={form} alg={int(alg)}", flush=True)
+    ref = pyoracle.run_pb("pb_linear", s)
+    # inactive systems are untouched
+    assert np.array_equal(ref["X"][:, (ns - 5) * ups:], s["X"][:, (ns - 5) * ups:])
+    for spb, shm in ((1, 0), (2, 1), (5, 0)):
+        got = run_pb_gpu(s, spb=spb, shared_matrix=shm)
+        assert_pb_bitwise(got, ref, f"spb={spb} shared_matrix={shm}")
+
+
+@pytest.mark.parametrize("alg", [Algorithms.RKCK45, Algorithms.RK4], ids=["rkck45", "rk4"])
+def test_dense_duffing_ring_256(oracle, alg):
+    import pyoracle
+    spec = W.config4_ring(ns=24, ups=256, periods=1, algorithm=alg)
+    if alg == Algorithms.RK4:
+        spec["options"]["dt0"] = 2.0 * W.PI / 400.0
+    print(f"dense ring 256 alg={int(alg)}", flush=True)
+    ref = pyoracle.run_pb("ring_duffing", spec)
+    for shm in (0, 1):
+        got = run_pb_gpu(spec, shared_matrix=shm)
+        assert_pb_bitwise(got, ref, f"shared_matrix={shm}")
+
+
+def test_second_solve_continues(oracle):
+    """A second Solve after moving the time domain continues from the stored
+    state, as two oracle runs do."""
+    import pyoracle
+    spec = W.config4_ring(ns=8, ups=64, periods=1)
+    s = PBProblemSolver("ring_duffing", 8, 64, Algorithms.RKCK45)
+    try:
+        W.load_pb(s, spec)
+        s.solve_blocking()
+        first = W.read_pb(s)
+        td = s.host_array(PB.TimeDomain)
+        td[0] = td[1]
+        td[1] = 2.0 * td[1]
+        s.SynchroniseFromHostToDevice(PB.TimeDomain)
+        s.solve_blocking()
+        second = W.read_pb(s)
+    finally:
+        s.close()
+    ref1 = pyoracle.run_pb("ring_duffing", spec)
+    assert_pb_bitwise(first, ref1, "first")
+    spec2 = dict(spec)
+    spec2["X"], spec2["T"] = ref1["X"], ref1["T"]
+    spec2["TD"] = np.stack([spec["TD"][1], 2.0 * spec["TD"][1]])
+    ref2 = pyoracle.run_pb("ring_duffing", spec2)
+    assert_pb_bitwise(second, ref2, "second")
