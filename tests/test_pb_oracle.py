@@ -251,3 +251,27 @@ def test_tutorial_ring_dw_event_termination(oracle):
         assert (dts[:-1] > 1e-6).all() and dts[-1] > 0
         assert o["DOt"][n_do - 1, j] == o["T"][j]
         assert np.array_equal(o["DOx"][n_do - 1, :, sl], o["X"][:, sl])
+
+
+def test_host_reference_values(oracle):
+    """End states that the reference solver itself produced (its device code
+    compiled for the host): the per-thread T1 double-well Duffing at k = 0.2
+    and the per-block T1 ring with 8 identical units.  Agreement to 1e-13
+    (the reference build used glibc cos/pow, this oracle its own routines)."""
+    import json
+    import os
+    import pyoracle
+    import scenarios
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "host_reference_values.json")))
+    pt = gold["per_thread_T1_duffing_doublewell"]
+    spec = dict(next(s for n, _, s in scenarios.all_scenarios() if n == "tutorial_duffing_doublewell"))
+    spec = W.subset(spec, [0])
+    assert spec["cPAR"][0, 0] == 0.2
+    o = pyoracle.run("duffing_dw", spec)
+    assert abs(o["X"][0, 0] - pt["x_end"]) < 1e-13
+    pb = gold["per_block_T1_ring_8_units"]
+    s = W.tutorial_ring_dw(ns=1, ups=8)
+    s["rtol"] = s["atol"] = 1e-10
+    o = pyoracle.run_pb("ring_dw", s)
+    assert o["T"][0] == pytest.approx(pb["t_end"], abs=1e-5)
+    assert np.abs(o["X"][0] - pb["x0_end"]).max() < 1e-13
