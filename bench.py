@@ -1,21 +1,41 @@
 """Benchmark of the MI355X MPGOS solver path.
 
-Workload (one GPU): the Lorenz ensemble, 262144 instances, rho ~ U[0, 21],
-x0 = (10, 10, 10) + N(0, 0.01), adaptive RKCK45 at rtol = atol = 1e-10 over
-t in [0, 10], FP64.  One "step" is one Solve() over the whole batch with the
-inputs already resident in HBM: a device-to-device reset of the initial state
-followed by the integration kernel.  The metric is accepted instance-steps per
-second (one accepted Runge-Kutta step of one instance).
+Default workload (one GPU): the Lorenz ensemble, 262144 instances,
+rho ~ U[0, 21], x0 = (10, 10, 10) + N(0, 0.01), adaptive RKCK45 at
+rtol = atol = 1e-10 over t in [0, 10], FP64.  One "step" is one Solve() over
+the whole batch with the inputs already resident in HBM: a device-to-device
+reset of the initial state followed by the integration kernel.  The metric is
+accepted instance-steps per second (one accepted Runge-Kutta step of one
+instance).
 
-Printed (rank 0, one JSON line): the contract fields plus
-  roofline      -- algorithmic FP64 FLOP/s of the integration kernel (FLOPs per
-                   attempted step from the oracle's op count x attempted steps,
-                   divided by the kernel time measured with HIP events on the
-                   solver's stream) against the 78.6 TFLOP/s FP64 peak;
-  cpu_baseline  -- the FP64 OpenMP oracle on the host cores, same workload;
-  parity        -- the GPU result compared bit for bit with that oracle run;
-  forms         -- the plain kernel in the reference's launch shape and the
-                   configuration chosen by the autotuner (mpgos/tuning.py).
+Other workloads (--workload), each printed as the same JSON line:
+  duffing_rk4     16384 forced Duffing oscillators, fixed-step RK4,
+                  dt = 2pi/1000 over 64 forcing periods (64001 steps each);
+  duffing_uneven  131072 Duffing oscillators with forcing frequency
+                  eps ~ 10^U[-3,0] and horizons of 16..4096 periods, RKCK45 at
+                  1e-9, event x' = 0 with up to 256 logged hits (one Solve is
+                  long: tens of seconds);
+  duffing_adjoint 65536 Duffing oscillators with a quadrature state, RK4 at
+                  dt = 2pi/500 (or --algorithm rkck45 at 1e-9): forward Solve
+                  and discrete adjoint (record + reverse), reporting the
+                  adjoint/forward cost ratio;
+  ring            4096 per-block rings of 256 coupled Duffing units with a
+                  dense coupling matrix, RKCK45 at 1e-9 over 4 periods; the
+                  coupling product on FP64 MFMA (metric: accepted
+                  system-steps/s).
+
+Printed (one JSON line): the contract fields plus
+  roofline      -- algorithmic FLOP/s of the dominant kernel (FLOPs per
+                   attempted step from the oracle's op count, or the stated
+                   per-step count for the ring, x attempted steps, divided by
+                   the kernel time measured with HIP events on the solver's
+                   stream) against the MI355X FP64 peak (78.6 TFLOP/s for both
+                   the vector ALUs and the matrix cores);
+  cpu_baseline  -- the FP64 OpenMP oracle on this host's cores, on the full
+                   workload or a stated bounded sample of it;
+  parity        -- the GPU result compared bit for bit with that oracle run.
+--copy-calibration MiB runs plain device-to-device copies (for scaling HBM
+counters) and prints the bytes moved.
 """
 import argparse
 import ctypes
@@ -30,14 +50,29 @@ sys.path.insert(0, os.path.join(ROOT, "massively-parallel-gpu-ode-solver_amd"))
 import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+ENGINE_NOTE = "FP64 VALU (MI355X FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s)"
 
 
-def time_form(mpgos, spec, cfg, steps, warmup):
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def threads_here():
+    return len(os.sched_getaffinity(0))
+
+
+def bitwise(a, b, keys):
+    return all(np.array_equal(np.asarray(a[k]).view(np.uint8), np.asarray(b[k]).view(np.uint8)) for k in keys)
+
+
+# --------------------------------------------------------------------------- per-thread workloads
+def time_form(mpgos, spec, cfg, steps, warmup, precision=0):
     """K timed steps (device-side input reset + Solve) of one launch configuration."""
     from mpgos import workloads as W
     from mpgos.tuning import DeviceSnapshot, apply
-    s = mpgos.ProblemSolver(spec["system"], spec["nt"], spec["algorithm"])
+    s = mpgos.ProblemSolver(spec["system"], spec["nt"], spec["algorithm"], precision, ndo=spec.get("ndo", 0))
     try:
         W.load(s, spec)
         apply(s, cfg)
@@ -48,11 +83,13 @@ def time_form(mpgos, spec, cfg, steps, warmup):
         snap.sync()
         kernel_ms = []
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for i in range(steps):
             snap.restore()
             s.Solve()
             snap.sync()
             kernel_ms.append(s.last_solve_ms())
+            if steps > 1 and kernel_ms[-1] > 2000:
+                log(f"step {i}: kernel {kernel_ms[-1]:.1f} ms")
         t1 = time.perf_counter()
         snap.close()
         s.SynchroniseFromDeviceToHost(mpgos.ListOfVariables.All)
@@ -65,43 +102,49 @@ def time_form(mpgos, spec, cfg, steps, warmup):
             "kernel_ms_min": float(np.min(kernel_ms)), "out": out, "config": cfg}
 
 
-def tune(mpgos, spec):
+def tune(mpgos, spec, precision=0):
     from mpgos import workloads as W
     from mpgos.tuning import autotune
-    s = mpgos.ProblemSolver(spec["system"], spec["nt"], spec["algorithm"])
+    s = mpgos.ProblemSolver(spec["system"], spec["nt"], spec["algorithm"], precision, ndo=spec.get("ndo", 0))
     try:
         W.load(s, spec)
-        best, table = autotune(s, log=lambda m: print(m, file=sys.stderr))
+        best, _ = autotune(s, log=log)
     finally:
         s.close()
-    return best, table
+    return best
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nt", type=int, default=262144)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the solver path has no inter-GPU work "
-              "(instances shard by index range with zero exchange); run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+def flops_per_attempt(pyoracle, W, spec, sample_idx):
+    sample = W.subset(spec, sample_idx)
+    fl = pyoracle.run(spec["system"], sample, count_flops=True)
+    return fl["flops"] / float(fl["nacc"].sum() + fl["nrej"].sum())
 
-    import mpgos
-    from mpgos import workloads as W
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-    spec = W.config1_lorenz(nt=args.nt)
-    forms = {"plain": time_form(mpgos, spec, {"mode": 0, "block": 64}, args.steps, args.warmup)}
-    if not args.no_tune:
-        best_cfg, table = tune(mpgos, spec)
-        forms["autotuned"] = time_form(mpgos, spec, best_cfg, args.steps, args.warmup)
+def per_thread(args, mpgos, W, pyoracle):
+    prec = {"f64": 0, "f32": 1, "f32k": 2}[args.precision]
+    if args.workload == "lorenz":
+        spec = W.config1_lorenz(nt=args.nt or 262144)
+        name = f"lorenz_rkck45_{spec['nt']}"
+        cpu_idx = None                                  # full workload on the CPU
+        desc = {"system": "lorenz", "algorithm": "RKCK45", "rtol": 1e-10, "atol": 1e-10, "t_end": 10.0}
+    elif args.workload == "duffing_rk4":
+        spec = W.config0_duffing_rk4(nt=args.nt or 16384)
+        name = f"duffing_rk4_{spec['nt']}"
+        cpu_idx = np.arange(0, spec["nt"], 16)          # every 16th instance
+        desc = {"system": "duffing", "algorithm": "RK4", "dt": "2pi/1000", "periods": 64}
     else:
-        forms["persistent"] = time_form(mpgos, spec, {"mode": 1, "block": 256, "round": 32}, args.steps, args.warmup)
+        spec = W.config2_duffing_uneven(nt=args.nt or 131072)
+        name = f"duffing_uneven_rkck45_{spec['nt']}"
+        cpu_idx = np.arange(0, spec["nt"], 2048)        # stratified: every 2048th instance
+        desc = {"system": "duffing_eps", "algorithm": "RKCK45", "rtol": 1e-9, "atol": 1e-9, "events": "x'=0, 256 logged"}
+    forms = {}
+    if args.form in ("auto", "plain"):
+        forms["plain"] = time_form(mpgos, spec, {"mode": 0, "block": 64}, args.steps, args.warmup, prec)
+    if args.form in ("auto", "persistent"):
+        forms["persistent"] = time_form(mpgos, spec, {"mode": 1, "block": 256, "round": 32}, args.steps, args.warmup, prec)
+    if args.form == "auto" and not args.no_tune:
+        best_cfg = tune(mpgos, spec, prec)
+        forms["autotuned"] = time_form(mpgos, spec, best_cfg, args.steps, args.warmup, prec)
     best = min(forms, key=lambda k: forms[k]["wall_ms_per_step"])
     res = forms[best]
     out = res["out"]
@@ -109,60 +152,260 @@ def main():
     attempted = accepted + int(out["nrej"].sum())
     value = accepted / (res["wall_ms_per_step"] * 1e-3)
 
-    # algorithmic FLOPs per attempted step: oracle op count over a sample
-    import pyoracle
-    sample = W.subset(spec, np.arange(0, args.nt, max(1, args.nt // 256)))
-    fl = pyoracle.run("lorenz", sample, count_flops=True)
-    flops_per_attempt = fl["flops"] / float(fl["nacc"].sum() + fl["nrej"].sum())
-    flops_per_launch = flops_per_attempt * attempted
-    achieved_tflops = flops_per_launch / (res["kernel_ms"] * 1e-3) / 1e12
+    nt = spec["nt"]
+    fpa = flops_per_attempt(pyoracle, W, spec, np.arange(0, nt, max(1, nt // (16 if args.workload != "lorenz" else 256))))
+    achieved = fpa * attempted / (res["kernel_ms"] * 1e-3) / 1e12
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "lorenz_rkck45_262144_traffic.json")
+    cpu = parity = None
+    if not args.no_cpu:
+        sub = spec if cpu_idx is None else W.subset(spec, cpu_idx)
+        ref = pyoracle.run(spec["system"], sub, nthreads=threads_here())
+        what = "full workload" if cpu_idx is None else f"{sub['nt']} of {nt} instances (every {nt // sub['nt']}th)"
+        cpu = {"value": float(ref["nacc"].sum() / ref["seconds"]), "unit": "instance-steps/s", "cores": threads_here(),
+               "kind": "port",
+               "sample": f"{what}: {int(ref['nacc'].sum())} accepted steps in {ref['seconds']:.2f} s, FP64 OpenMP oracle"}
+        mine = out if cpu_idx is None else {k: (v[..., cpu_idx] if isinstance(v, np.ndarray) and v.ndim and v.shape[-1] == nt else v)
+                                            for k, v in out.items()}
+        if prec == 0:
+            parity = {"bitwise_equal_to_oracle": bool(bitwise(mine, ref, ("X", "T", "nacc", "nrej", "status"))),
+                      "instances_compared": int(sub["nt"])}
+        else:
+            scale = np.maximum(np.abs(ref["X"]), 1e-3)
+            parity = {"max_rel_state_error_vs_fp64_oracle": float(np.max(np.abs(mine["X"] - ref["X"]) / scale)),
+                      "median_rel_state_error": float(np.median(np.abs(mine["X"] - ref["X"]) / scale)),
+                      "accepted_step_count_mismatches": int(np.sum(mine["nacc"] != ref["nacc"])),
+                      "instances_compared": int(sub["nt"])}
+    line = contract_line(args, value, "instance-steps/s", res["wall_ms_per_step"],
+                         {"workload": name, "instances": nt, **desc, "kernel_form": best, "launch": res["config"],
+                          "accepted_steps_per_solve": accepted, "rejected_steps_per_solve": attempted - accepted},
+                         {"bound": "mfma", "engine": ENGINE_NOTE, "achieved": achieved,
+                          "peak": FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": achieved / (FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS),
+                          "traffic": traffic_for(name), "flops_per_attempted_step": fpa, "kernel_ms": res["kernel_ms"]},
+                         cpu, parity)
+    line["forms"] = {k: {"config": v["config"], "wall_ms_per_step": v["wall_ms_per_step"], "kernel_ms": v["kernel_ms"],
+                         "instance_steps_per_s": accepted / (v["wall_ms_per_step"] * 1e-3)} for k, v in forms.items()}
+    if "plain" in forms and "persistent" in forms:
+        line["persistent_over_plain"] = forms["plain"]["kernel_ms"] / forms["persistent"]["kernel_ms"]
+    return line
+
+
+# --------------------------------------------------------------------------- adjoint
+def adjoint(args, mpgos, W, pyoracle):
+    from mpgos.enums import Algorithms, ListOfVariables as V
+    from mpgos.tuning import DeviceSnapshot
+    alg = Algorithms.RK4 if args.algorithm == "rk4" else Algorithms.RKCK45
+    spec = W.config3_duffing_adjoint(nt=args.nt or 65536, algorithm=alg)
+    nt = spec["nt"]
+    K = args.checkpoint_interval
+    s = mpgos.ProblemSolver(spec["system"], nt, alg)
+    try:
+        W.load(s, spec)
+        snap = DeviceSnapshot(s)
+        lam = np.zeros((3, nt))
+        lam[2] = 1.0                                    # dL/dq(H): d/dq of the quadrature state
+        fwd, rec, rev = [], [], []
+        steps_needed = None
+        for i in range(args.warmup + args.steps):
+            snap.restore()
+            s.Solve()
+            snap.sync()
+            f_ms = s.last_solve_ms()
+            if steps_needed is None:
+                s.SynchroniseFromDeviceToHost(V.All)
+                s.InsertSynchronisationPoint()
+                s.SynchroniseSolver()
+                out = W.read(s)
+                steps_needed = int(out["nacc"].max())
+                cap = steps_needed // K + 2
+            snap.restore()
+            s.adjoint_record(K, cap)
+            snap.sync()
+            r_ms = s.last_solve_ms()
+            s.host_array(V.AdjointState)[...] = lam
+            s.SynchroniseFromHostToDevice(V.AdjointState)
+            s.adjoint_reverse()
+            snap.sync()
+            b_ms = s.last_solve_ms()
+            if i >= args.warmup:
+                fwd.append(f_ms); rec.append(r_ms); rev.append(b_ms)
+        snap.close()
+    finally:
+        s.close()
+    f, r, b = float(np.mean(fwd)), float(np.mean(rec)), float(np.mean(rev))
+    accepted = int(out["nacc"].sum())
+    attempted = accepted + int(out["nrej"].sum())
+    fpa = flops_per_attempt(pyoracle, W, spec, np.arange(0, nt, nt // 16))
+    achieved = fpa * attempted / (f * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        idx = np.arange(0, nt, 64)
+        ref = pyoracle.run(spec["system"], W.subset(spec, idx), nthreads=threads_here())
+        cpu = {"value": float(ref["nacc"].sum() / ref["seconds"]), "unit": "instance-steps/s", "cores": threads_here(),
+               "kind": "port", "sample": f"forward pass of {len(idx)} of {nt} instances (every 64th): "
+                                         f"{int(ref['nacc'].sum())} accepted steps in {ref['seconds']:.2f} s"}
+    line = contract_line(args, accepted / (f * 1e-3), "instance-steps/s", f,
+                         {"workload": f"duffing_adjoint_{args.algorithm}_{nt}", "instances": nt,
+                          "algorithm": args.algorithm.upper(), "checkpoint_interval": K,
+                          "accepted_steps_per_solve": accepted},
+                         {"bound": "mfma", "engine": ENGINE_NOTE, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                          "flops_per_attempted_step": fpa, "kernel_ms": f},
+                         cpu, None)
+    line["adjoint"] = {"forward_ms": f, "record_ms": r, "reverse_ms": b, "adjoint_over_forward": (r + b) / f,
+                       "reverse_over_forward": b / f}
+    return line
+
+
+# --------------------------------------------------------------------------- per-block ring
+RING_FLOPS_NOTE = ("per system-step: 6 stages x 2 x U^2 (coupling) + U x 404 (unit right-hand sides with cos at 38, "
+                   "stage combinations, error, tolerance) = 889,856 at U = 256")
+
+
+def ring(args, mpgos, W, pyoracle):
+    from mpgos.perblock import PBOption, PBProblemSolver
+    ns = args.nt or 4096
+    U = 256
+    spec = W.config4_ring(ns=ns, ups=U, periods=4)
+    kernels = {"mfma": 2, "lane": 1} if args.form == "auto" else {args.form: 2 if args.form == "mfma" else 1}
+    forms = {}
+    for name, k in kernels.items():
+        s = PBProblemSolver("ring_duffing", ns, U, spec["algorithm"])
+        try:
+            s.SolverOption(PBOption.CouplingKernel, k)
+            times = []
+            for i in range(args.warmup + args.steps):
+                W.load_pb(s, spec)
+                s.Solve()
+                s.SynchroniseDevice()
+                if i >= args.warmup:
+                    times.append(s.last_solve_ms())
+            s.SynchroniseFromDeviceToHost(0)
+            s.InsertSynchronisationPoint()
+            s.SynchroniseSolver()
+            forms[name] = {"kernel_ms": float(np.mean(times)), "out": W.read_pb(s)}
+        finally:
+            s.close()
+    best = min(forms, key=lambda k: forms[k]["kernel_ms"])
+    res = forms[best]
+    out = res["out"]
+    accepted = int(out["nacc"].sum())
+    attempted = accepted + int(out["nrej"].sum())
+    flops_step = 6 * 2 * U * U + U * 404
+    achieved = flops_step * attempted / (res["kernel_ms"] * 1e-3) / 1e12
+    cpu = parity = None
+    if not args.no_cpu:
+        idx = np.arange(0, ns, max(1, ns // 64))
+        sub = W.subset_pb(spec, idx)
+        ref = pyoracle.run_pb("ring_duffing", sub, nthreads=threads_here())
+        cpu = {"value": float(ref["nacc"].sum() / ref["seconds"]), "unit": "system-steps/s", "cores": threads_here(),
+               "kind": "port", "sample": f"{len(idx)} of {ns} systems (every {ns // len(idx)}th): "
+                                         f"{int(ref['nacc'].sum())} accepted system-steps in {ref['seconds']:.2f} s"}
+        mine = W.subset_pb_out(out, idx, U)
+        parity = {"bitwise_equal_to_oracle": bool(bitwise(mine, ref, ("X", "T", "nacc", "nrej", "status"))),
+                  "systems_compared": len(idx)}
+        if len(forms) > 1:
+            parity["mfma_equals_lane_kernel_bitwise"] = bool(bitwise(forms["mfma"]["out"], forms["lane"]["out"],
+                                                                     ("X", "T", "nacc", "nrej")))
+    line = contract_line(args, accepted / (res["kernel_ms"] * 1e-3), "system-steps/s", res["kernel_ms"],
+                         {"workload": f"ring_duffing_{ns}x{U}", "systems": ns, "units_per_system": U,
+                          "algorithm": "RKCK45", "rtol": 1e-9, "atol": 1e-9, "periods": 4, "kernel_form": best,
+                          "accepted_system_steps": accepted},
+                         {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic_for(f"ring_duffing_{ns}x{U}"),
+                          "flops_per_attempted_step": flops_step, "flops_note": RING_FLOPS_NOTE,
+                          "kernel_ms": res["kernel_ms"]},
+                         cpu, parity)
+    line["forms"] = {k: {"kernel_ms": v["kernel_ms"], "system_steps_per_s": accepted / (v["kernel_ms"] * 1e-3)}
+                     for k, v in forms.items()}
+    return line
+
+
+# --------------------------------------------------------------------------- common
+def traffic_for(name):
+    tfile = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
 
-    cpu = None
-    parity = None
-    if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        ref = pyoracle.run("lorenz", spec, nthreads=threads)
-        cpu = {"value": float(ref["nacc"].sum() / ref["seconds"]), "unit": "instance-steps/s",
-               "cores": threads, "kind": "port",
-               "sample": f"full workload ({args.nt} instances, {int(ref['nacc'].sum())} accepted steps), "
-                         f"{ref['seconds']:.2f} s, FP64 OpenMP oracle schedule(dynamic,64)"}
-        same = all(np.array_equal(np.asarray(out[k]).view(np.uint8), np.asarray(ref[k]).view(np.uint8))
-                   for k in ("X", "T", "nacc", "nrej", "status"))
-        parity = {"bitwise_equal_to_oracle": bool(same), "instances": args.nt}
 
+def contract_line(args, value, unit, ms_per_step, config, roofline, cpu, parity):
     line = {
         "metric": "ODE instance-steps/s at rtol 1e-9 vs FP64 oracle; adjoint/forward cost",
         "value": value,
-        "unit": "instance-steps/s",
+        "unit": unit,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": res["wall_ms_per_step"],
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded SplitMix64, seed 20261202, stream 1)",
-        "config": {"workload": f"lorenz_rkck45_{args.nt}", "system": "lorenz", "algorithm": "RKCK45",
-                   "instances": args.nt, "rtol": 1e-10, "atol": 1e-10, "t_end": 10.0,
-                   "kernel_form": best, "launch": res["config"], "accepted_steps_per_solve": accepted,
-                   "rejected_steps_per_solve": attempted - accepted},
-        "roofline": {"bound": "fp64-valu", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "flops_per_attempted_step": flops_per_attempt, "kernel_ms": res["kernel_ms"]},
+        "dtype": {"f64": "f64", "f32": "f32", "f32k": "f32 (Kahan-compensated state and time)"}[args.precision],
+        "data": "synthetic (seeded SplitMix64, seed 20261202)",
+        "config": config,
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,
-        "forms": {k: {"config": v["config"], "wall_ms_per_step": v["wall_ms_per_step"], "kernel_ms": v["kernel_ms"],
-                      "instance_steps_per_s": accepted / (v["wall_ms_per_step"] * 1e-3)} for k, v in forms.items()},
     }
     if cpu:
         line["speedup_vs_cpu"] = value / cpu["value"]
+    return line
+
+
+def copy_calibration(mib):
+    from mpgos.tuning import hip
+    h = hip()
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if h.hipMalloc(ctypes.byref(a), n) != 0 or h.hipMalloc(ctypes.byref(b), n) != 0:
+        raise MemoryError("hipMalloc failed")
+    h.hipDeviceSynchronize.argtypes = []
+    for _ in range(3):
+        if h.hipMemcpy(b, a, n, 3) != 0:
+            raise RuntimeError("copy failed")
+    h.hipDeviceSynchronize()
+    h.hipFree(a)
+    h.hipFree(b)
+    print(json.dumps({"copy_calibration": {"bytes_per_copy": n, "copies": 3,
+                                           "note": "device-to-device hipMemcpy: reads and writes bytes_per_copy each"}}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="lorenz",
+                    choices=["lorenz", "duffing_rk4", "duffing_uneven", "duffing_adjoint", "ring"])
+    ap.add_argument("--nt", type=int, default=0, help="instances (systems for ring); 0 = the workload's size")
+    ap.add_argument("--form", default="auto", help="auto | plain | persistent (per-thread); auto | mfma | lane (ring)")
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "f32k"])
+    ap.add_argument("--algorithm", default="rk4", choices=["rk4", "rkck45"], help="duffing_adjoint only")
+    ap.add_argument("--checkpoint-interval", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the solver path has no inter-GPU work "
+              "(instances and systems shard by index range with zero exchange); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+
+    import mpgos
+    from mpgos import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if args.workload in ("lorenz", "duffing_rk4", "duffing_uneven"):
+        line = per_thread(args, mpgos, W, pyoracle)
+    elif args.workload == "duffing_adjoint":
+        line = adjoint(args, mpgos, W, pyoracle)
+    else:
+        line = ring(args, mpgos, W, pyoracle)
     print(json.dumps(line))
 
 

@@ -275,6 +275,12 @@ enum {
     MPGOS_PB_DENSE_OUTPUT_MINIMUM_TIME_STEP,
     MPGOS_PB_SHARED_GLOBAL_VARIABLES,
     MPGOS_PB_SHARED_COUPLING_MATRICES,
+    /* New (no reference counterpart): which coupling kernel runs.  0 automatic
+     * (the dense-contraction MFMA kernel for a full FP64 matrix with UPS a
+     * multiple of 32 in [32, 256], else lane per pair), 1 lane per pair (any
+     * storage form; SPB as created), 2 dense contraction (8 systems per
+     * workgroup, 2*UPS threads).  Results are bit-identical between them. */
+    MPGOS_PB_COUPLING_KERNEL,
     MPGOS_PB_NUM_OPTIONS
 };
 

@@ -1,5 +1,6 @@
 // Per-block coupled-systems solver for gfx950: one workgroup integrates SPB
-// systems of UPS coupled units; one lane owns one (system, unit) pair.
+// systems of UPS coupled units; a lane owns PAIRS (system, unit) pairs of one
+// unit (PAIRS = 1 in the lane-per-pair form, 2 in the dense-contraction form).
 //
 // Semantics follow the reference's per-block kernels
 // (SourceCodes/CoupledSystmes_PerBlock_Solver.cuh, single-system form at
@@ -19,14 +20,31 @@
 //
 // MI355X design: the reductions are LDS atomics on ordered bit patterns
 // (min of non-negative doubles, AND of flags), which are order-independent and
-// therefore deterministic; coupling terms are double-buffered in LDS so one
-// barrier per stage suffices; the coupling matrix is read from L2 (or staged
-// in LDS when SharedCouplingMatrices is on); multiply-adds are explicit fma in
-// the contracted form of the reference's expressions, so the host oracle
-// reproduces every bit.
+// therefore deterministic; multiply-adds are explicit fma in the contracted
+// form of the reference's expressions, so the host oracle reproduces every bit.
+//
+// Two coupling forms:
+//  * lane-per-pair (any storage form): each lane forms its own dot product,
+//    coupling matrix from L2 (or LDS), coupling terms double-buffered in LDS
+//    (one barrier per stage);
+//  * dense contraction (full matrix, FP64, UPS a multiple of 32 up to 256, 8
+//    systems per workgroup, 4 pairs per lane): the products C . [CPT_s0 .. CPT_s7] of all 8 systems run on
+//    v_mfma_f64_16x16x4f64.  Wave w owns unit tiles w, w + waves, ...  (two
+//    chains at a time); A is the
+//    16 x 4 tile M[16 t + i][4 c + k] (lane i + 16 k, from L2), B the 4 x 16
+//    tile CPT[4 c + k][system j] (lane j + 16 k, from LDS, columns 8..15
+//    zero), and the accumulator chains the 64 column blocks in order.  The
+//    instruction accumulates its four products as a chain of fused
+//    multiply-adds in k order, so the result equals the sequential
+//    fma(M[u][col], CPT[col], v) over col = 0 .. UPS-1 of the lane-per-pair
+//    form bit for bit.  Output element (row i, column j) sits in lane
+//    j + 16 (i mod 4), register i / 4; it goes through LDS to the lane that
+//    owns the pair (two barriers per stage).
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "mpgos.h"
 #include "mpgos_math.h"
@@ -37,6 +55,10 @@ namespace pb {
 template <int N> struct Pos { static constexpr int value = N > 0 ? N : 1; };
 
 constexpr int kMaxBlock = 1024;
+constexpr int kMfmaSystems = 8;          // systems per workgroup of the dense-contraction form
+constexpr int kMfmaPairs = 4;            // pairs per lane of the dense-contraction form (2 * UPS threads)
+
+enum Coupling { kLaneCoupling = 0, kMfmaCoupling = 1 };
 
 template <class Sys, class P>
 struct Args {
@@ -87,28 +109,43 @@ template <class P> __device__ __forceinline__ P dmax(P a, P b);
 template <> __device__ __forceinline__ double dmax<double>(double a, double b) { return __builtin_fmax(a, b); }
 template <> __device__ __forceinline__ float dmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
 
-// LDS bytes: system scopes, two coupling-term buffers, optional global
-// parameters and coupling matrix copies.
+// LDS bytes: system scopes, coupling terms (two [system][unit][coupling]
+// buffers in the lane-per-pair form; one [coupling][unit][16] B-operand
+// buffer plus a [coupling][unit][system] result buffer in the dense form),
+// optional global parameters and coupling matrix copies.
+struct LdsLayout {
+    size_t cpt, coup, g, gi, m, total;
+};
+
 template <class Sys, class P>
-__host__ __device__ inline size_t lds_layout(int SPB, int UPS, int sharedGlobals, int sharedMatrix, int cMatSize,
-                                             size_t* offCPT, size_t* offG, size_t* offGi, size_t* offM)
+__host__ __device__ inline LdsLayout lds_layout(int cpl, int SPB, int UPS, int sharedGlobals, int sharedMatrix,
+                                                int cMatSize)
 {
-    size_t o = 0;
-    o += (size_t)SPB * sizeof(SysScope<Sys, P>);
+    constexpr int NCp = Pos<Sys::NC>::value;
+    LdsLayout L;
+    size_t o = (size_t)SPB * sizeof(SysScope<Sys, P>);
     o = (o + 15) & ~(size_t)15;
-    *offCPT = o;
-    o += 2 * (size_t)SPB * UPS * Pos<Sys::NC>::value * sizeof(P);
-    *offG = o;
+    L.cpt = o;
+    if (cpl == kMfmaCoupling) {
+        o += (size_t)NCp * UPS * 16 * sizeof(P);
+        L.coup = o;
+        o += (size_t)NCp * UPS * SPB * sizeof(P);
+    } else {
+        o += 2 * (size_t)SPB * UPS * NCp * sizeof(P);
+        L.coup = o;
+    }
+    L.g = o;
     if (sharedGlobals) o += (size_t)Sys::NGP * sizeof(P);
     o = (o + 15) & ~(size_t)15;
-    *offM = o;
+    L.m = o;
     if (sharedMatrix) o += (size_t)Sys::NC * cMatSize * sizeof(P);
-    *offGi = o;
+    L.gi = o;
     if (sharedGlobals) o += (size_t)Sys::NiGP * sizeof(int);
-    return o;
+    L.total = o;
+    return L;
 }
 
-template <class Sys, class P, int ALG>
+template <class Sys, class P>
 struct Unit {
     static constexpr int UD = Sys::UD, NC = Sys::NC, NE = Sys::NE;
     using A = Args<Sys, P>;
@@ -143,61 +180,127 @@ struct Unit {
                     v = math::fma(Mi[d], cptSys[col * NC + i], v);
                 }
             }
-            const int ci = cIndex[i];
-            const P sc = strength[i] * CPF[i];
-#pragma unroll
-            for (int c = 0; c < UD; c++)
-                if (c == ci) F[c] = math::fma(sc, v, F[c]);
+            add(cIndex[i], strength[i] * CPF[i], v, F);
         }
+    }
+
+    static __device__ __forceinline__ void add(int ci, P sc, P v, P* F)
+    {
+#pragma unroll
+        for (int c = 0; c < UD; c++)
+            if (c == ci) F[c] = math::fma(sc, v, F[c]);
     }
 };
 
-// One workgroup = SPB systems.  Lane (lsys, u) with lsys = tid / UPS, u = tid % UPS.
-template <class Sys, class P, int ALG>
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+// Dense contraction of the coupling terms of the block's systems: for every
+// coupling i, unit u and system s < SPB,
+//   coup[(i * UPS + u) * SPB + s] = sum_col M_i[u + col * UPS] * cptB[(i * UPS + col) * 16 + s]
+// accumulated in column order.  Called by every lane of the workgroup
+// (v_mfma needs all 64 lanes of a wave: blockDim.x is a multiple of 64); UPS is
+// a multiple of 16.
+template <int NC>
+__device__ __forceinline__ void mfma_contract(const double* M, const double* cptB, double* coup, int UPS, int SPB)
+{
+    constexpr int G = 8;                                  // column blocks per load group (UPS % 32 == 0)
+    const int l = threadIdx.x & 63;
+    const int nw = blockDim.x >> 6;
+    const int tiles = UPS >> 4;
+    const int i16 = l & 15, k4 = l >> 4;
+    const int blocks = UPS >> 2;
+    // two unit tiles per pass (two independent accumulation chains per wave)
+    for (int t0 = threadIdx.x >> 6; t0 < tiles; t0 += 2 * nw) {
+        const int t1 = t0 + nw;
+        const bool two = t1 < tiles;                      // wave-uniform
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            const double* A0 = M + (size_t)i * UPS * UPS + t0 * 16 + i16 + (size_t)k4 * UPS;
+            const double* A1 = M + (size_t)i * UPS * UPS + (two ? t1 : t0) * 16 + i16 + (size_t)k4 * UPS;
+            const double* Bi = cptB + (size_t)i * UPS * 16 + k4 * 16 + i16;
+            mfma_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+            for (int c0 = 0; c0 < blocks; c0 += G) {
+                double a0[G], a1[G], b[G];
+#pragma unroll
+                for (int d = 0; d < G; d++) {
+                    a0[d] = A0[(size_t)(c0 + d) * 4 * UPS];
+                    a1[d] = A1[(size_t)(c0 + d) * 4 * UPS];
+                    b[d] = Bi[(c0 + d) * 64];
+                }
+#pragma unroll
+                for (int d = 0; d < G; d++) {
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[d], b[d], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[d], b[d], acc1, 0, 0, 0);
+                }
+            }
+            if (i16 < SPB) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    coup[((size_t)i * UPS + t0 * 16 + k4 + 4 * r) * SPB + i16] = acc0[r];
+                    if (two) coup[((size_t)i * UPS + t1 * 16 + k4 + 4 * r) * SPB + i16] = acc1[r];
+                }
+            }
+        }
+    }
+}
+
+// One workgroup = SPB systems.  A system has UPS / PAIRS lanes; lane t belongs
+// to block system lsys = t / (UPS / PAIRS) and owns the PAIRS units
+// u0 + p * (UPS / PAIRS), u0 = t % (UPS / PAIRS), so the system-scope scalars
+// (time, step) are shared by the lane's pairs and global accesses to unit-scope
+// arrays stay coalesced.
+template <class Sys, class P, int ALG, int PAIRS, int CPL>
 __device__ void pb_body(const Args<Sys, P>& a)
 {
     constexpr int UD = Sys::UD, NC = Sys::NC, NUP = Sys::NUP, NSP = Sys::NSP, NGP = Sys::NGP, NiGP = Sys::NiGP;
     constexpr int NUA = Sys::NUA, NiUA = Sys::NiUA, NSA = Sys::NSA, NiSA = Sys::NiSA, NE = Sys::NE;
+    constexpr int NCp = Pos<NC>::value;
     using S = SysScope<Sys, P>;
+    using U = Unit<Sys, P>;
     using math::fma;
 
     extern __shared__ __align__(16) unsigned char mpgos_pb_lds[];
-    size_t offCPT, offG, offGi, offM;
-    lds_layout<Sys, P>(a.SPB, a.UPS, a.sharedGlobals, a.sharedMatrix, a.cMatSize, &offCPT, &offG, &offGi, &offM);
+    const LdsLayout L = lds_layout<Sys, P>(CPL, a.SPB, a.UPS, a.sharedGlobals, a.sharedMatrix, a.cMatSize);
     S* sys = reinterpret_cast<S*>(mpgos_pb_lds);
-    P* cptBuf = reinterpret_cast<P*>(mpgos_pb_lds + offCPT);
-    const int cptStride = a.SPB * a.UPS * Pos<NC>::value;
+    P* cptBuf = reinterpret_cast<P*>(mpgos_pb_lds + L.cpt);
+    P* coupBuf = reinterpret_cast<P*>(mpgos_pb_lds + L.coup);
+    const int cptStride = a.SPB * a.UPS * NCp;
 
-    const int UPS = a.UPS, NS = a.NS;
+    const int UPS = a.UPS, NS = a.NS, SPB = a.SPB;
+    const int lanesPerSys = UPS / PAIRS;
     const int tid = threadIdx.x;
-    const int lsys = tid / UPS;
-    const int u = tid - lsys * UPS;
-    const bool lane = tid < a.SPB * UPS;                       // lane owns a (system, unit) pair
-    const int gsys = blockIdx.x * a.SPB + lsys;
+    const int lsys = tid / lanesPerSys;
+    const int u0 = tid - lsys * lanesPerSys;
+    const bool lane = lsys < SPB;                              // lane owns PAIRS (system, unit) pairs
+    const bool leader = lane && u0 == 0;
+    const size_t NU = (size_t)NS * UPS;
+    const int gsys = blockIdx.x * SPB + lsys;
     const bool exists = lane && gsys < NS;
     const bool active = exists && gsys < a.nActive;
-    const bool leader = lane && u == 0;
-    const size_t NU = (size_t)NS * UPS;
-    const size_t gu = (size_t)gsys * UPS + u;                 // unit-scope index
+    S* const my = lane ? &sys[lsys] : &sys[0];
+    auto uid = [&](int p) { return u0 + p * lanesPerSys; };
+    auto gu = [&](int p) { return (size_t)gsys * UPS + uid(p); };   // unit-scope index
 
     // ---- global scope (LDS when SharedGlobalVariables / SharedCouplingMatrices)
     P* gPAR = a.gPAR;
     int* gPARi = a.gPARi;
     const P* M = a.cMat;
     if (a.sharedGlobals) {
-        P* lg = reinterpret_cast<P*>(mpgos_pb_lds + offG);
-        int* lgi = reinterpret_cast<int*>(mpgos_pb_lds + offGi);
+        P* lg = reinterpret_cast<P*>(mpgos_pb_lds + L.g);
+        int* lgi = reinterpret_cast<int*>(mpgos_pb_lds + L.gi);
         for (int i = tid; i < NGP; i += blockDim.x) lg[i] = a.gPAR[i];
         for (int i = tid; i < NiGP; i += blockDim.x) lgi[i] = a.gPARi[i];
         gPAR = lg;
         gPARi = lgi;
     }
     if (a.sharedMatrix) {
-        P* lm = reinterpret_cast<P*>(mpgos_pb_lds + offM);
+        P* lm = reinterpret_cast<P*>(mpgos_pb_lds + L.m);
         for (int i = tid; i < NC * a.cMatSize; i += blockDim.x) lm[i] = a.cMat[i];
         M = lm;
     }
-    int cIndex[Pos<NC>::value];
+    if (CPL == kMfmaCoupling)   // B operand: columns of absent systems and 8..15 stay zero
+        for (int i = tid; i < NCp * UPS * 16; i += blockDim.x) cptBuf[i] = P(0);
+    int cIndex[NCp];
 #pragma unroll
     for (int i = 0; i < NC; i++) cIndex[i] = a.cIndex[i];
 
@@ -221,29 +324,39 @@ __device__ void pb_body(const Args<Sys, P>& a)
     }
 
     // ---- unit scope in registers
-    P X[UD], uPAR[Pos<NUP>::value], uACC[Pos<NUA>::value], EF[Pos<NE>::value], CPF[Pos<NC>::value];
-    int uACCi[Pos<NiUA>::value];
-    if (exists) {
+    P X[PAIRS][UD], uPAR[PAIRS][Pos<NUP>::value], uACC[PAIRS][Pos<NUA>::value], EF[PAIRS][Pos<NE>::value];
+    P CPF[PAIRS][NCp];
+    int uACCi[PAIRS][Pos<NiUA>::value];
 #pragma unroll
-        for (int c = 0; c < UD; c++) X[c] = a.X[gu + c * NU];
+    for (int p = 0; p < PAIRS; p++) {
+        if (exists) {
+            const size_t g = gu(p);
 #pragma unroll
-        for (int c = 0; c < NUP; c++) uPAR[c] = a.uPAR[gu + c * NU];
+            for (int c = 0; c < UD; c++) X[p][c] = a.X[g + c * NU];
 #pragma unroll
-        for (int c = 0; c < NUA; c++) uACC[c] = a.uACC[gu + c * NU];
+            for (int c = 0; c < NUP; c++) uPAR[p][c] = a.uPAR[g + c * NU];
 #pragma unroll
-        for (int c = 0; c < NiUA; c++) uACCi[c] = a.uACCi[gu + c * NU];
+            for (int c = 0; c < NUA; c++) uACC[p][c] = a.uACC[g + c * NU];
+#pragma unroll
+            for (int c = 0; c < NiUA; c++) uACCi[p][c] = a.uACCi[g + c * NU];
+        }
+#pragma unroll
+        for (int i = 0; i < NC; i++) CPF[p][i] = P(0);
     }
-#pragma unroll
-    for (int i = 0; i < NC; i++) CPF[i] = P(0);
     __syncthreads();
 
-    S* my = lane ? &sys[lsys] : &sys[0];
-#define MPGOS_PB_USER(TT, DTT) TT, DTT, my->TD, X, uPAR, my->sPAR, gPAR, gPARi, uACC, uACCi, my->sACC, my->sACCi
+#define MPGOS_PB_USER(p, TT, DTT) TT, DTT, my->TD, X[p], uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p], my->sACC, my->sACCi
     // ---- initialisation (PB-SOL:2465-2528)
-    if (active) Sys::template Initialization<P>(gsys, u, my->DOidx, MPGOS_PB_USER(my->t, my->dt));
+    if (active) {
+#pragma unroll
+        for (int p = 0; p < PAIRS; p++) Sys::template Initialization<P>(gsys, uid(p), my->DOidx, MPGOS_PB_USER(p, my->t, my->dt));
+    }
     __syncthreads();
     if (NE > 0) {
-        if (active) Sys::template EventFunction<P>(gsys, u, EF, MPGOS_PB_USER(my->t, my->dt));
+        if (active) {
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++) Sys::template EventFunction<P>(gsys, uid(p), EF[p], MPGOS_PB_USER(p, my->t, my->dt));
+        }
         __syncthreads();
     }
     if (a.NDO > 0) {
@@ -251,8 +364,10 @@ __device__ void pb_body(const Args<Sys, P>& a)
         if (active && my->DOidx < a.NDO) {
             const int k = my->DOidx;
 #pragma unroll
-            for (int c = 0; c < UD; c++) a.DOx[gu + c * NU + (size_t)k * NU * UD] = X[c];
-            if (u == 0) a.DOt[gsys + (size_t)k * NS] = my->t;
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) a.DOx[gu(p) + c * NU + (size_t)k * NU * UD] = X[p][c];
+            if (leader) a.DOt[gsys + (size_t)k * NS] = my->t;
         }
         __syncthreads();
         if (leader && active && my->DOidx < a.NDO) {
@@ -264,16 +379,46 @@ __device__ void pb_body(const Args<Sys, P>& a)
     }
 
     int buf = 0;
-    auto rhs = [&](P* F, P* Y, P T) {
-        // RHS of this unit, coupling terms published, one barrier, coupling added
-        P* cpt = cptBuf + (size_t)buf * cptStride;
-        if (lane && my->term == 0)
-            Sys::template OdeFunction<P>(gsys, u, F, Y, T, uPAR, my->sPAR, gPAR, gPARi, uACC, uACCi, my->sACC, my->sACCi,
-                                         cpt + (size_t)(lsys * UPS + u) * Pos<NC>::value, CPF);
-        __syncthreads();
-        if (lane && my->term == 0)
-            Unit<Sys, P, ALG>::couple(a, M, cIndex, my->strength, cpt + (size_t)lsys * UPS * Pos<NC>::value, u, CPF, F);
-        buf ^= 1;
+    // right-hand sides of this lane's pairs at (Y[p], T), then the coupling terms
+    auto rhs = [&](P (*F)[UD], P (*Y)[UD], const P T) {
+        const bool run = lane && my->term == 0;
+        if constexpr (CPL == kMfmaCoupling) {
+            if (run) {
+#pragma unroll
+                for (int p = 0; p < PAIRS; p++) {
+                    P cp[NCp];
+                    Sys::template OdeFunction<P>(gsys, uid(p), F[p], Y[p], T, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
+                                                 my->sACC, my->sACCi, cp, CPF[p]);
+#pragma unroll
+                    for (int i = 0; i < NC; i++) cptBuf[((size_t)i * UPS + uid(p)) * 16 + lsys] = cp[i];
+                }
+            }
+            __syncthreads();
+            mfma_contract<NC>(M, cptBuf, coupBuf, UPS, SPB);
+            __syncthreads();
+            if (run) {
+#pragma unroll
+                for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                    for (int i = 0; i < NC; i++)
+                        U::add(cIndex[i], my->strength[i] * CPF[p][i], coupBuf[((size_t)i * UPS + uid(p)) * SPB + lsys], F[p]);
+            }
+        } else {
+            P* cpt = cptBuf + (size_t)buf * cptStride;
+            if (run) {
+#pragma unroll
+                for (int p = 0; p < PAIRS; p++)
+                    Sys::template OdeFunction<P>(gsys, uid(p), F[p], Y[p], T, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
+                                                 my->sACC, my->sACCi, cpt + (size_t)(lsys * UPS + uid(p)) * NCp, CPF[p]);
+            }
+            __syncthreads();
+            if (run) {
+#pragma unroll
+                for (int p = 0; p < PAIRS; p++)
+                    U::couple(a, M, cIndex, my->strength, cpt + (size_t)lsys * UPS * NCp, uid(p), CPF[p], F[p]);
+            }
+            buf ^= 1;
+        }
     };
 
     for (;;) {
@@ -288,61 +433,83 @@ __device__ void pb_body(const Args<Sys, P>& a)
         }
         __syncthreads();
         const P h = my->dt, t = my->t;
-        P next[UD], err[UD];
+        P next[PAIRS][UD], err[PAIRS][UD];
         bool isfin = true;
         if (ALG == MPGOS_RK4) {
             // PB-STP:2016-2224
-            P k[UD], Y[UD];
+            P k[PAIRS][UD], Y[PAIRS][UD];
             rhs(next, X, t);
 #pragma unroll
-            for (int c = 0; c < UD; c++) Y[c] = fma(next[c] * P(0.5), h, X[c]);
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) Y[p][c] = fma(next[p][c] * P(0.5), h, X[p][c]);
             const P t2 = t + P(0.5) * h;
             rhs(k, Y, t2);
 #pragma unroll
-            for (int c = 0; c < UD; c++) { next[c] = fma(P(2), k[c], next[c]); Y[c] = fma(k[c] * P(0.5), h, X[c]); }
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) { next[p][c] = fma(P(2), k[p][c], next[p][c]); Y[p][c] = fma(k[p][c] * P(0.5), h, X[p][c]); }
             rhs(k, Y, t2);
 #pragma unroll
-            for (int c = 0; c < UD; c++) { next[c] = fma(P(2.0), k[c], next[c]); Y[c] = fma(k[c], h, X[c]); }
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) { next[p][c] = fma(P(2.0), k[p][c], next[p][c]); Y[p][c] = fma(k[p][c], h, X[p][c]); }
             rhs(k, Y, t + h);
 #pragma unroll
-            for (int c = 0; c < UD; c++) {
-                next[c] = fma(h * (next[c] + k[c]), P(1.0 / 6.0), X[c]);
-                if (!math::isfinite(next[c])) isfin = false;
-            }
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) {
+                    next[p][c] = fma(h * (next[p][c] + k[p][c]), P(1.0 / 6.0), X[p][c]);
+                    if (!math::isfinite(next[p][c])) isfin = false;
+                }
         } else {
             // PB-STP:2227-2544
-            P k1[UD], k2[UD], k3[UD], k4[UD], k5[UD], k6[UD], Y[UD];
+            P k1[PAIRS][UD], k2[PAIRS][UD], k3[PAIRS][UD], k4[PAIRS][UD], k5[PAIRS][UD], k6[PAIRS][UD], Y[PAIRS][UD];
             rhs(k1, X, t);
 #pragma unroll
-            for (int c = 0; c < UD; c++) Y[c] = fma(k1[c] * P(1.0 / 5.0), h, X[c]);
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) Y[p][c] = fma(k1[p][c] * P(1.0 / 5.0), h, X[p][c]);
             rhs(k2, Y, fma(P(1.0 / 5.0), h, t));
 #pragma unroll
-            for (int c = 0; c < UD; c++) Y[c] = fma(fma(k2[c], P(9.0 / 40.0), k1[c] * P(3.0 / 40.0)), h, X[c]);
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) Y[p][c] = fma(fma(k2[p][c], P(9.0 / 40.0), k1[p][c] * P(3.0 / 40.0)), h, X[p][c]);
             rhs(k3, Y, fma(P(3.0 / 10.0), h, t));
 #pragma unroll
-            for (int c = 0; c < UD; c++)
-                Y[c] = fma(fma(k3[c], P(6.0 / 5.0), fma(k2[c], P(-9.0 / 10.0), k1[c] * P(3.0 / 10.0))), h, X[c]);
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++)
+                    Y[p][c] = fma(fma(k3[p][c], P(6.0 / 5.0), fma(k2[p][c], P(-9.0 / 10.0), k1[p][c] * P(3.0 / 10.0))), h, X[p][c]);
             rhs(k4, Y, fma(P(3.0 / 5.0), h, t));
 #pragma unroll
-            for (int c = 0; c < UD; c++)
-                Y[c] = fma(fma(k4[c], P(35.0 / 27.0), fma(k3[c], P(-70.0 / 27.0), fma(k2[c], P(5.0 / 2.0), k1[c] * P(-11.0 / 54.0)))), h, X[c]);
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++)
+                    Y[p][c] = fma(fma(k4[p][c], P(35.0 / 27.0), fma(k3[p][c], P(-70.0 / 27.0), fma(k2[p][c], P(5.0 / 2.0),
+                                  k1[p][c] * P(-11.0 / 54.0)))), h, X[p][c]);
             rhs(k5, Y, t + h);
 #pragma unroll
-            for (int c = 0; c < UD; c++)
-                Y[c] = fma(fma(k5[c], P(253.0 / 4096.0), fma(k4[c], P(44275.0 / 110592.0), fma(k3[c], P(575.0 / 13824.0),
-                           fma(k2[c], P(175.0 / 512.0), k1[c] * P(1631.0 / 55296.0))))), h, X[c]);
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++)
+                    Y[p][c] = fma(fma(k5[p][c], P(253.0 / 4096.0), fma(k4[p][c], P(44275.0 / 110592.0), fma(k3[p][c], P(575.0 / 13824.0),
+                                  fma(k2[p][c], P(175.0 / 512.0), k1[p][c] * P(1631.0 / 55296.0))))), h, X[p][c]);
             rhs(k6, Y, fma(P(7.0 / 8.0), h, t));
 #pragma unroll
-            for (int c = 0; c < UD; c++) {
-                next[c] = fma(fma(k6[c], P(512.0 / 1771.0), fma(k4[c], P(125.0 / 594.0), fma(k3[c], P(250.0 / 621.0), k1[c] * P(37.0 / 378.0)))), h, X[c]);
-                P e = k1[c] * P(37.0 / 378.0 - 2825.0 / 27648.0);
-                e = fma(k3[c], P(250.0 / 621.0 - 18575.0 / 48384.0), e);
-                e = fma(k4[c], P(125.0 / 594.0 - 13525.0 / 55296.0), e);
-                e = fma(k5[c], P(0.0 - 277.0 / 14336.0), e);
-                e = fma(k6[c], P(512.0 / 1771.0 - 1.0 / 4.0), e);
-                err[c] = fma(h, math::fabs(e), P(1e-18));
-                if (!math::isfinite(next[c]) || !math::isfinite(err[c])) isfin = false;
-            }
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) {
+                    next[p][c] = fma(fma(k6[p][c], P(512.0 / 1771.0), fma(k4[p][c], P(125.0 / 594.0), fma(k3[p][c], P(250.0 / 621.0),
+                                     k1[p][c] * P(37.0 / 378.0)))), h, X[p][c]);
+                    P e = k1[p][c] * P(37.0 / 378.0 - 2825.0 / 27648.0);
+                    e = fma(k3[p][c], P(250.0 / 621.0 - 18575.0 / 48384.0), e);
+                    e = fma(k4[p][c], P(125.0 / 594.0 - 13525.0 / 55296.0), e);
+                    e = fma(k5[p][c], P(0.0 - 277.0 / 14336.0), e);
+                    e = fma(k6[p][c], P(512.0 / 1771.0 - 1.0 / 4.0), e);
+                    err[p][c] = fma(h, math::fabs(e), P(1e-18));
+                    if (!math::isfinite(next[p][c]) || !math::isfinite(err[p][c])) isfin = false;
+                }
         }
         if (lane && my->term == 0 && !isfin) my->isfin = 0;
         // ---- controller (PB-CTL:424-542)
@@ -351,11 +518,13 @@ __device__ void pb_body(const Args<Sys, P>& a)
                 P R = P(1e30);
                 int upd = 1;
 #pragma unroll
-                for (int c = 0; c < UD; c++) {
-                    const P tol = dmax<P>(a.rtol[c] * dmax<P>(math::fabs(next[c]), math::fabs(X[c])), a.atol[c]);
-                    upd = upd & (err[c] < tol);
-                    R = dmin<P>(R, tol / err[c]);
-                }
+                for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                    for (int c = 0; c < UD; c++) {
+                        const P tol = dmax<P>(a.rtol[c] * dmax<P>(math::fabs(next[p][c]), math::fabs(X[p][c])), a.atol[c]);
+                        upd = upd & (err[p][c] < tol);
+                        R = dmin<P>(R, tol / err[p][c]);
+                    }
                 atomicAnd(&my->update, upd);
                 atomicMin(&my->ratioBits, order_bits<P>(R));
             }
@@ -390,22 +559,27 @@ __device__ void pb_body(const Args<Sys, P>& a)
         }
         __syncthreads();
         // ---- events (PB-SOL:2618-2655, PB-EVT:215-268)
-        P EFn[Pos<NE>::value];
+        P EFn[PAIRS][Pos<NE>::value];
         if (NE > 0) {
-            P tn = my->t + my->dt;
-            // the event function of the candidate state, at the end of the step
-            if (lane && my->term == 0)
-                Sys::template EventFunction<P>(gsys, u, EFn, tn, my->newDt, my->TD, next, uPAR, my->sPAR, gPAR, gPARi, uACC,
-                                               uACCi, my->sACC, my->sACCi);
-            if (lane && my->update == 1 && my->term == 0) {
+            const P tn = my->t + my->dt;
+            if (lane && my->term == 0) {
                 bool corr = false;
                 P ev = my->dt;
 #pragma unroll
-                for (int i = 0; i < NE; i++) {
-                    const P tol = a.etol[i];
-                    if ((EF[i] > tol && EFn[i] < -tol && a.edir[i] <= 0) || (EF[i] < -tol && EFn[i] > tol && a.edir[i] >= 0)) {
-                        ev = dmin<P>(ev, -EF[i] / (EFn[i] - EF[i]) * my->dt);
-                        corr = true;
+                for (int p = 0; p < PAIRS; p++) {
+                    // the event function of the candidate state, at the end of the step
+                    Sys::template EventFunction<P>(gsys, uid(p), EFn[p], tn, my->newDt, my->TD, next[p], uPAR[p], my->sPAR, gPAR,
+                                                   gPARi, uACC[p], uACCi[p], my->sACC, my->sACCi);
+                    if (my->update == 1) {
+#pragma unroll
+                        for (int i = 0; i < NE; i++) {
+                            const P tol = a.etol[i];
+                            if ((EF[p][i] > tol && EFn[p][i] < -tol && a.edir[i] <= 0) ||
+                                (EF[p][i] < -tol && EFn[p][i] > tol && a.edir[i] >= 0)) {
+                                ev = dmin<P>(ev, -EF[p][i] / (EFn[p][i] - EF[p][i]) * my->dt);
+                                corr = true;
+                            }
+                        }
                     }
                 }
                 if (corr) {
@@ -425,38 +599,51 @@ __device__ void pb_body(const Args<Sys, P>& a)
         const int upd = my->update;
         if (lane && upd == 1) {
 #pragma unroll
-            for (int c = 0; c < UD; c++) X[c] = next[c];
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) X[p][c] = next[p][c];
         }
         __syncthreads();
         if (leader && upd == 1) my->t += my->dt;
         __syncthreads();
-        if (lane && upd == 1) Sys::template ActionAfterSuccessfulTimeStep<P>(gsys, u, my->udt, MPGOS_PB_USER(my->t, my->dt));
+        if (lane && upd == 1) {
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+                Sys::template ActionAfterSuccessfulTimeStep<P>(gsys, uid(p), my->udt, MPGOS_PB_USER(p, my->t, my->dt));
+        }
         __syncthreads();
         if (NE > 0) {
             if (lane && upd == 1) {
 #pragma unroll
-                for (int i = 0; i < NE; i++) {
-                    const P tol = a.etol[i];
-                    if ((EF[i] > tol && math::fabs(EFn[i]) < tol && a.edir[i] <= 0) ||
-                        (EF[i] < -tol && math::fabs(EFn[i]) < tol && a.edir[i] >= 0))
-                        Sys::template ActionAfterEventDetection<P>(gsys, u, i, my->udt, MPGOS_PB_USER(my->t, my->dt));
-                }
+                for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                    for (int i = 0; i < NE; i++) {
+                        const P tol = a.etol[i];
+                        if ((EF[p][i] > tol && math::fabs(EFn[p][i]) < tol && a.edir[i] <= 0) ||
+                            (EF[p][i] < -tol && math::fabs(EFn[p][i]) < tol && a.edir[i] >= 0))
+                            Sys::template ActionAfterEventDetection<P>(gsys, uid(p), i, my->udt, MPGOS_PB_USER(p, my->t, my->dt));
+                    }
             }
-            __syncthreads();
-            if (lane && upd == 1) Sys::template EventFunction<P>(gsys, u, EFn, MPGOS_PB_USER(my->t, my->newDt));
             __syncthreads();
             if (lane && upd == 1) {
 #pragma unroll
-                for (int i = 0; i < NE; i++) EF[i] = EFn[i];
+                for (int p = 0; p < PAIRS; p++) Sys::template EventFunction<P>(gsys, uid(p), EFn[p], MPGOS_PB_USER(p, my->t, my->newDt));
+            }
+            __syncthreads();
+            if (lane && upd == 1) {
+#pragma unroll
+                for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                    for (int i = 0; i < NE; i++) EF[p][i] = EFn[p][i];
             }
         }
         // ---- dense output (PB-DO:312-395)
         if (a.NDO > 0) {
             if (leader) {
                 if (upd == 1) {
-                    int s = (my->DOidx < a.NDO && my->doT < my->t && my->skipped >= a.doFreq - 1) ? 1 : 0;
-                    if (my->DOidx < a.NDO && (my->end == 1 || my->udt == 1)) s = 1;
-                    my->updDO = s;
+                    int st = (my->DOidx < a.NDO && my->doT < my->t && my->skipped >= a.doFreq - 1) ? 1 : 0;
+                    if (my->DOidx < a.NDO && (my->end == 1 || my->udt == 1)) st = 1;
+                    my->updDO = st;
                 } else {
                     my->updDO = 0;
                 }
@@ -465,8 +652,10 @@ __device__ void pb_body(const Args<Sys, P>& a)
             if (lane && my->updDO == 1 && upd == 1) {
                 const int k = my->DOidx;
 #pragma unroll
-                for (int c = 0; c < UD; c++) a.DOx[gu + c * NU + (size_t)k * NU * UD] = X[c];
-                if (u == 0) a.DOt[gsys + (size_t)k * NS] = my->t;
+                for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                    for (int c = 0; c < UD; c++) a.DOx[gu(p) + c * NU + (size_t)k * NU * UD] = X[p][c];
+                if (leader) a.DOt[gsys + (size_t)k * NS] = my->t;
             }
             __syncthreads();
             if (leader && upd == 1) {
@@ -490,21 +679,28 @@ __device__ void pb_body(const Args<Sys, P>& a)
         }
         __syncthreads();
         int done = 1;
-        for (int s = 0; s < a.SPB; s++) done &= sys[s].term;
+        for (int q = 0; q < SPB; q++) done &= sys[q].term;
         if (done) break;
         __syncthreads();
     }
     __syncthreads();
     // ---- finalisation and write-back (PB-SOL:2814-2887)
-    if (active) Sys::template Finalization<P>(gsys, u, my->DOidx, MPGOS_PB_USER(my->t, my->dt));
+    if (active) {
+#pragma unroll
+        for (int p = 0; p < PAIRS; p++) Sys::template Finalization<P>(gsys, uid(p), my->DOidx, MPGOS_PB_USER(p, my->t, my->dt));
+    }
     __syncthreads();
     if (exists) {
 #pragma unroll
-        for (int c = 0; c < UD; c++) a.X[gu + c * NU] = X[c];
+        for (int p = 0; p < PAIRS; p++) {
+            const size_t g = gu(p);
 #pragma unroll
-        for (int c = 0; c < NUA; c++) a.uACC[gu + c * NU] = uACC[c];
+            for (int c = 0; c < UD; c++) a.X[g + c * NU] = X[p][c];
 #pragma unroll
-        for (int c = 0; c < NiUA; c++) a.uACCi[gu + c * NU] = uACCi[c];
+            for (int c = 0; c < NUA; c++) a.uACC[g + c * NU] = uACC[p][c];
+#pragma unroll
+            for (int c = 0; c < NiUA; c++) a.uACCi[g + c * NU] = uACCi[p][c];
+        }
     }
     if (leader && active) {
         S& s = sys[lsys];
@@ -520,10 +716,21 @@ __device__ void pb_body(const Args<Sys, P>& a)
 #undef MPGOS_PB_USER
 }
 
-template <class Sys, class P, int ALG>
-__global__ void __launch_bounds__(kMaxBlock) pb_kernel(Args<Sys, P> a)
+// Lane-per-pair form; MAXT bounds the block size so that blocks of up to 256
+// threads get the larger register budget.
+template <class Sys, class P, int ALG, int MAXT>
+__global__ void __launch_bounds__(MAXT) pb_kernel(Args<Sys, P> a)
 {
-    pb_body<Sys, P, ALG>(a);
+    pb_body<Sys, P, ALG, 1, kLaneCoupling>(a);
+}
+
+// Dense-contraction form: SPB = 8, 4 pairs per lane, 2 * UPS threads (UPS a
+// multiple of 32 up to 256, so every wave is full).
+template <class Sys, class P, int ALG>
+__global__ void __launch_bounds__(512) pb_mfma_kernel(Args<Sys, P> a)
+{
+    if constexpr (std::is_same<P, double>::value)
+        pb_body<Sys, P, ALG, kMfmaPairs, kMfmaCoupling>(a);
 }
 
 } // namespace pb

@@ -58,6 +58,8 @@ public:
     size_t ldsMax = 0;
 
     int nActive = 0, sharedGlobals = 1, sharedMatrix = 0, doFreq = 1;
+    int couplingKernel = 0;   // 0 automatic, 1 lane-per-pair, 2 dense contraction (MFMA)
+    int lastForm = -1;        // coupling form of the last launch
     double dt0 = 1e-2, dtmax = 1.0e6, dtmin = 1.0e-12, grow = 5.0, shrink = 0.1, doMinDt = 0.0;
     std::vector<double> rtol, atol, etol;
     std::vector<int> edir;
@@ -158,6 +160,10 @@ public:
         case MPGOS_PB_DENSE_OUTPUT_MINIMUM_TIME_STEP: doMinDt = v; return MPGOS_OK;
         case MPGOS_PB_SHARED_GLOBAL_VARIABLES: sharedGlobals = (int)v != 0; return MPGOS_OK;
         case MPGOS_PB_SHARED_COUPLING_MATRICES: sharedMatrix = (int)v != 0; return MPGOS_OK;
+        case MPGOS_PB_COUPLING_KERNEL:
+            if ((int)v < 0 || (int)v > 2) { set_error("CouplingKernel must be 0 (automatic), 1 (lane per pair) or 2 (dense contraction)"); return MPGOS_E_ARG; }
+            couplingKernel = (int)v;
+            return MPGOS_OK;
         default:
             set_error("SolverOption: unknown or not applicable option " + std::to_string(opt));
             return MPGOS_E_ARG;
@@ -378,10 +384,23 @@ public:
         return a;
     }
 
-    size_t lds_bytes(int sm) const
+    // the dense-contraction (MFMA) form applies to full FP64 matrices with UPS a
+    // multiple of 32 in [32, 256] (2 * UPS threads fill whole waves);
+    // CouplingKernel 0 picks it when it applies
+    bool mfma_applies() const
     {
-        size_t o1, o2, o3, o4;
-        return lds_layout<Sys, P>(SPB, UPS, sharedGlobals, sm, cMatSize, &o1, &o2, &o3, &o4);
+        return std::is_same<P, double>::value && CCI == 0 && CBW == 0 && UPS % 32 == 0 && UPS >= 32 && UPS <= 256;
+    }
+    int coupling_form() const
+    {
+        if (couplingKernel == 1) return kLaneCoupling;
+        if (couplingKernel == 2) return kMfmaCoupling;
+        return mfma_applies() ? kMfmaCoupling : kLaneCoupling;
+    }
+
+    size_t lds_bytes(int cpl, int spb, int sm) const
+    {
+        return lds_layout<Sys, P>(cpl, spb, UPS, sharedGlobals, sm, cMatSize).total;
     }
 
     // PB-IF:1949-1965
@@ -389,15 +408,31 @@ public:
     {
         MPGOS_HIP_TRY(hipSetDevice(device));
         Args<Sys, P> a = make_args();
+        const int cpl = coupling_form();
+        if (cpl == kMfmaCoupling && !mfma_applies()) {
+            set_error("per-block: the dense-contraction coupling kernel needs a full FP64 coupling matrix and UPS a multiple of 32 up to 256");
+            return MPGOS_E_ARG;
+        }
+        const int spb = cpl == kMfmaCoupling ? kMfmaSystems : SPB;
+        a.SPB = spb;
         // the coupling matrix goes to LDS only when asked for and when it fits
-        a.sharedMatrix = sharedMatrix && lds_bytes(1) <= ldsMax ? 1 : 0;
-        const size_t lds = lds_bytes(a.sharedMatrix);
+        a.sharedMatrix = sharedMatrix && lds_bytes(cpl, spb, 1) <= ldsMax ? 1 : 0;
+        const size_t lds = lds_bytes(cpl, spb, a.sharedMatrix);
         if (lds > ldsMax) { set_error("per-block: system scope and coupling terms exceed the LDS of a workgroup"); return MPGOS_E_ARG; }
-        const int threads = ((SPB * UPS + 63) / 64) * 64;
-        const int grid = NS / SPB + (NS % SPB == 0 ? 0 : 1);
+        const int threads = cpl == kMfmaCoupling ? kMfmaSystems * UPS / kMfmaPairs : ((SPB * UPS + 63) / 64) * 64;
+        const int grid = NS / spb + (NS % spb == 0 ? 0 : 1);
+        lastForm = cpl;
         MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
-        if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4>), dim3(grid), dim3(threads), lds, strm, a);
-        else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45>), dim3(grid), dim3(threads), lds, strm, a);
+        if (cpl == kMfmaCoupling) {
+            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RK4>), dim3(grid), dim3(threads), lds, strm, a);
+            else hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RKCK45>), dim3(grid), dim3(threads), lds, strm, a);
+        } else if (threads <= 256) {
+            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4, 256>), dim3(grid), dim3(threads), lds, strm, a);
+            else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, 256>), dim3(grid), dim3(threads), lds, strm, a);
+        } else {
+            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a);
+            else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a);
+        }
         MPGOS_HIP_TRY(hipGetLastError());
         MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
         return MPGOS_OK;

@@ -59,6 +59,7 @@ class PBOption(enum.IntEnum):
     DenseOutputMinimumTimeStep = 11
     SharedGlobalVariables = 12
     SharedCouplingMatrices = 13
+    CouplingKernel = 14          # new: 0 automatic, 1 lane per pair, 2 dense contraction (MFMA)
 
 
 class _PBDims(ctypes.Structure):

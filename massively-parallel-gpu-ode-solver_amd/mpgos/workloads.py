@@ -314,6 +314,41 @@ _PB_OPT = {"dt0": 0, "dtmax": 2, "dtmin": 3, "grow": 4, "shrink": 5, "do_freq": 
            "shared_globals": 12, "shared_matrix": 13}
 
 
+def subset_pb(spec, idx):
+    """A per-block spec restricted to the systems `idx`."""
+    idx = np.asarray(idx)
+    ups = spec["ups"]
+    units = (idx[:, None] * ups + np.arange(ups)[None, :]).ravel()
+    s = dict(spec)
+    s["ns"] = len(idx)
+    s["options"] = dict(spec["options"])
+    for k in ("X", "uPAR", "uACC", "uACCi"):
+        s[k] = np.ascontiguousarray(spec[k][:, units])
+    for k in ("TD", "sPAR", "sACC", "sACCi", "strength"):
+        s[k] = np.ascontiguousarray(spec[k][:, idx])
+    for k in ("T", "DOidx"):
+        s[k] = np.ascontiguousarray(spec[k][idx])
+    s.pop("n_active", None)
+    return s
+
+
+def subset_pb_out(out, idx, ups):
+    """Per-block outputs (read_pb) restricted to the systems `idx`."""
+    idx = np.asarray(idx)
+    units = (idx[:, None] * ups + np.arange(ups)[None, :]).ravel()
+    r = {}
+    for k, v in out.items():
+        if k in ("X", "uACC", "uACCi"):
+            r[k] = v[:, units]
+        elif k == "DOx":
+            r[k] = v[:, :, units]
+        elif k in ("TD", "sACC", "sACCi", "DOt", "sPAR", "strength"):
+            r[k] = v[:, idx]
+        elif k in ("T", "DOidx", "nacc", "nrej", "status"):
+            r[k] = v[idx]
+    return r
+
+
 def load_pb(solver, spec):
     """Options and inputs of a per-block spec into a PBProblemSolver, then H2D."""
     from .perblock import PB

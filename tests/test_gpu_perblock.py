@@ -8,6 +8,7 @@ import pytest
 
 from mpgos import workloads as W
 from mpgos.enums import Algorithms
+from mpgos._lib import MPGOSError
 from mpgos.perblock import PB, PBOption, PBProblemSolver
 
 pytestmark = pytest.mark.gpu
@@ -15,11 +16,13 @@ pytestmark = pytest.mark.gpu
 KEYS = ("X", "T", "TD", "uACC", "uACCi", "sACC", "sACCi", "DOidx", "nacc", "nrej", "status", "DOt", "DOx")
 
 
-def run_pb_gpu(spec, spb=None, shared_matrix=None, shared_globals=None, solves=1):
+def run_pb_gpu(spec, spb=None, shared_matrix=None, shared_globals=None, solves=1, kernel=None):
     s = PBProblemSolver(spec["system"], spec["ns"], spec["ups"], spec["algorithm"], spb=spb or spec.get("spb", 1),
                         cbw=spec["cbw"], cci=spec["cci"], ndo=spec["ndo"])
     try:
         W.load_pb(s, spec)
+        if kernel is not None:
+            s.SolverOption(PBOption.CouplingKernel, kernel)
         if shared_matrix is not None:
             s.SolverOption(PBOption.SharedCouplingMatrices, shared_matrix)
         if shared_globals is not None:
@@ -81,7 +84,7 @@ FORMS = [(0, 0), (0, 2), (1, 2), (1, 0)]
 def test_linear_ring_forms(oracle, form, alg):
     import pyoracle
     cci, cbw = form
-    ns, ups = 37, 24
+    ns, ups = 37, 32
     rs = W.Stream(W.SEED, 200 + 2 * cci + cbw)
     s = W.new_pb_spec("pb_linear", alg, ns, ups, cbw=cbw, cci=cci, ndo=8)
     Wd = 2 * cbw + 1 if cbw > 0 else ups
@@ -98,20 +101,55 @@ def test_linear_ring_forms(oracle, form, alg):
     # inactive systems are untouched
     assert np.array_equal(ref["X"][:, (ns - 5) * ups:], s["X"][:, (ns - 5) * ups:])
     for spb, shm in ((1, 0), (2, 1), (5, 0)):
-        got = run_pb_gpu(s, spb=spb, shared_matrix=shm)
-        assert_pb_bitwise(got, ref, f"spb={spb} shared_matrix={shm}")
+        got = run_pb_gpu(s, spb=spb, shared_matrix=shm, kernel=1)
+        assert_pb_bitwise(got, ref, f"lane kernel spb={spb} shared_matrix={shm}")
+    if form == (0, 0):
+        for shm in (0, 1):
+            got = run_pb_gpu(s, shared_matrix=shm, kernel=2)
+            assert_pb_bitwise(got, ref, f"mfma kernel shared_matrix={shm}")
+    else:
+        with pytest.raises(MPGOSError, match="dense-contraction"):
+            run_pb_gpu(s, kernel=2)
 
 
-@pytest.mark.parametrize("alg", [Algorithms.RKCK45, Algorithms.RK4], ids=["rkck45", "rk4"])
-def test_dense_duffing_ring_256(oracle, alg):
+def test_ring_dw_full_matrix_mfma(oracle):
+    """The tutorial's double-well ring with 32 units and its nearest-neighbour
+    coupling stored as a full matrix: events, accessories, user termination
+    and dense output through the dense-contraction kernel."""
     import pyoracle
-    spec = W.config4_ring(ns=24, ups=256, periods=1, algorithm=alg)
+    ns, ups = 11, 32
+    base = W.tutorial_ring_dw(ns=25, ups=ups)
+    s = W.new_pb_spec("ring_dw", Algorithms.RKCK45, ns, ups, ndo=64)
+    for k in ("rtol", "atol", "edir", "options", "gPAR", "cIndex"):
+        s[k] = base[k]
+    for key in ("TD", "T", "sPAR", "sACC", "sACCi", "strength", "DOidx"):
+        s[key] = np.ascontiguousarray(base[key][..., :ns])
+    for key in ("X", "uPAR", "uACC", "uACCi"):
+        s[key] = np.ascontiguousarray(base[key][:, :ns * ups])
+    ring = np.zeros((ups, ups))
+    for r in range(ups):
+        ring[r, (r + 1) % ups] = ring[r, (r - 1) % ups] = 1.0
+    s["cMat"] = np.concatenate([W.store_coupling(ring, 0, 0)] * 2)
+    print("ring_dw full matrix 32 units", flush=True)
+    ref = pyoracle.run_pb("ring_dw", s)
+    for kernel in (1, 2):
+        got = run_pb_gpu(s, kernel=kernel)
+        assert_pb_bitwise(got, ref, f"kernel={kernel}")
+    assert (ref["uACCi"] == 2).any() and (ref["T"] < s["TD"][1]).all()
+
+
+@pytest.mark.parametrize("kernel", [1, 2], ids=["lane", "mfma"])
+@pytest.mark.parametrize("alg", [Algorithms.RKCK45, Algorithms.RK4], ids=["rkck45", "rk4"])
+def test_dense_duffing_ring_256(oracle, alg, kernel):
+    import pyoracle
+    spec = W.config4_ring(ns=27, ups=256, periods=1, algorithm=alg)
     if alg == Algorithms.RK4:
         spec["options"]["dt0"] = 2.0 * W.PI / 400.0
-    print(f"dense ring 256 alg={int(alg)}", flush=True)
+    spec["n_active"] = 25
+    print(f"dense ring 256 alg={int(alg)} kernel={kernel}", flush=True)
     ref = pyoracle.run_pb("ring_duffing", spec)
     for shm in (0, 1):
-        got = run_pb_gpu(spec, shared_matrix=shm)
+        got = run_pb_gpu(spec, shared_matrix=shm, kernel=kernel)
         assert_pb_bitwise(got, ref, f"shared_matrix={shm}")
 
 
