@@ -281,6 +281,9 @@ enum {
      * storage form; SPB as created), 2 dense contraction (8 systems per
      * workgroup, 2*UPS threads).  Results are bit-identical between them. */
     MPGOS_PB_COUPLING_KERNEL,
+    /* New: units per lane of the dense-contraction kernel, 4 (2*UPS threads,
+     * the default) or 2 (4*UPS threads).  A launch shape only. */
+    MPGOS_PB_MFMA_UNITS_PER_LANE,
     MPGOS_PB_NUM_OPTIONS
 };
 

@@ -34,7 +34,8 @@
 #include "mpgos.h"
 #include "pt_engine.h"
 
-#ifndef MPGOS_EXTERNAL_ERROR_SINK
+#if !defined(MPGOS_EXTERNAL_ERROR_SINK) && !defined(MPGOS_FACADE_ERROR_SINK_DEFINED)
+#define MPGOS_FACADE_ERROR_SINK_DEFINED
 namespace mpgos {
 // Engine errors are reported the way the reference reports them (stderr).
 void set_error(const std::string& msg) { std::cerr << "ERROR: " << msg << std::endl; }

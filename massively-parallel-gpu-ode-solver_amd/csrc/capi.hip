@@ -27,6 +27,7 @@ static const Registration kSystems[] = {
 namespace pb {
 static const Registration kSystems[] = {
     {"ring_duffing", make_pb_ring_duffing},
+    {"ring_duffing_q", make_pb_ring_duffing_q},
     {"ring_dw", make_pb_ring_dw},
     {"pb_linear", make_pb_pb_linear},
 };

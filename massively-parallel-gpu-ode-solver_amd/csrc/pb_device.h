@@ -56,7 +56,7 @@ template <int N> struct Pos { static constexpr int value = N > 0 ? N : 1; };
 
 constexpr int kMaxBlock = 1024;
 constexpr int kMfmaSystems = 8;          // systems per workgroup of the dense-contraction form
-constexpr int kMfmaPairs = 4;            // pairs per lane of the dense-contraction form (2 * UPS threads)
+constexpr int kMfmaPairs = 4;            // default pairs per lane of the dense-contraction form (2 * UPS threads)
 
 enum Coupling { kLaneCoupling = 0, kMfmaCoupling = 1 };
 
@@ -724,13 +724,13 @@ __global__ void __launch_bounds__(MAXT) pb_kernel(Args<Sys, P> a)
     pb_body<Sys, P, ALG, 1, kLaneCoupling>(a);
 }
 
-// Dense-contraction form: SPB = 8, 4 pairs per lane, 2 * UPS threads (UPS a
-// multiple of 32 up to 256, so every wave is full).
-template <class Sys, class P, int ALG>
-__global__ void __launch_bounds__(512) pb_mfma_kernel(Args<Sys, P> a)
+// Dense-contraction form: SPB = 8, PAIRS units per lane, 8 * UPS / PAIRS
+// threads (UPS a multiple of 32 up to 256, so every wave is full).
+template <class Sys, class P, int ALG, int PAIRS>
+__global__ void __launch_bounds__(8 * 256 / PAIRS) pb_mfma_kernel(Args<Sys, P> a)
 {
     if constexpr (std::is_same<P, double>::value)
-        pb_body<Sys, P, ALG, kMfmaPairs, kMfmaCoupling>(a);
+        pb_body<Sys, P, ALG, PAIRS, kMfmaCoupling>(a);
 }
 
 } // namespace pb

@@ -60,6 +60,7 @@ public:
     int nActive = 0, sharedGlobals = 1, sharedMatrix = 0, doFreq = 1;
     int couplingKernel = 0;   // 0 automatic, 1 lane-per-pair, 2 dense contraction (MFMA)
     int lastForm = -1;        // coupling form of the last launch
+    int mfmaPairs = 4;        // units per lane of the dense-contraction form (2 or 4)
     double dt0 = 1e-2, dtmax = 1.0e6, dtmin = 1.0e-12, grow = 5.0, shrink = 0.1, doMinDt = 0.0;
     std::vector<double> rtol, atol, etol;
     std::vector<int> edir;
@@ -160,6 +161,10 @@ public:
         case MPGOS_PB_DENSE_OUTPUT_MINIMUM_TIME_STEP: doMinDt = v; return MPGOS_OK;
         case MPGOS_PB_SHARED_GLOBAL_VARIABLES: sharedGlobals = (int)v != 0; return MPGOS_OK;
         case MPGOS_PB_SHARED_COUPLING_MATRICES: sharedMatrix = (int)v != 0; return MPGOS_OK;
+        case MPGOS_PB_MFMA_UNITS_PER_LANE:
+            if ((int)v != 2 && (int)v != 4) { set_error("MfmaUnitsPerLane must be 2 or 4"); return MPGOS_E_ARG; }
+            mfmaPairs = (int)v;
+            return MPGOS_OK;
         case MPGOS_PB_COUPLING_KERNEL:
             if ((int)v < 0 || (int)v > 2) { set_error("CouplingKernel must be 0 (automatic), 1 (lane per pair) or 2 (dense contraction)"); return MPGOS_E_ARG; }
             couplingKernel = (int)v;
@@ -419,13 +424,16 @@ public:
         a.sharedMatrix = sharedMatrix && lds_bytes(cpl, spb, 1) <= ldsMax ? 1 : 0;
         const size_t lds = lds_bytes(cpl, spb, a.sharedMatrix);
         if (lds > ldsMax) { set_error("per-block: system scope and coupling terms exceed the LDS of a workgroup"); return MPGOS_E_ARG; }
-        const int threads = cpl == kMfmaCoupling ? kMfmaSystems * UPS / kMfmaPairs : ((SPB * UPS + 63) / 64) * 64;
+        const int threads = cpl == kMfmaCoupling ? kMfmaSystems * UPS / mfmaPairs : ((SPB * UPS + 63) / 64) * 64;
         const int grid = NS / spb + (NS % spb == 0 ? 0 : 1);
         lastForm = cpl;
         MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
-        if (cpl == kMfmaCoupling) {
-            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RK4>), dim3(grid), dim3(threads), lds, strm, a);
-            else hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RKCK45>), dim3(grid), dim3(threads), lds, strm, a);
+        if (cpl == kMfmaCoupling && mfmaPairs == 2) {
+            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RK4, 2>), dim3(grid), dim3(threads), lds, strm, a);
+            else hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RKCK45, 2>), dim3(grid), dim3(threads), lds, strm, a);
+        } else if (cpl == kMfmaCoupling) {
+            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RK4, 4>), dim3(grid), dim3(threads), lds, strm, a);
+            else hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RKCK45, 4>), dim3(grid), dim3(threads), lds, strm, a);
         } else if (threads <= 256) {
             if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4, 256>), dim3(grid), dim3(threads), lds, strm, a);
             else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, 256>), dim3(grid), dim3(threads), lds, strm, a);

@@ -36,6 +36,7 @@ struct Registration {
 SolverBase* make_pb_ring_duffing(int precision);
 SolverBase* make_pb_ring_dw(int precision);
 SolverBase* make_pb_pb_linear(int precision);
+SolverBase* make_pb_ring_duffing_q(int precision);
 } // namespace pb
 
 namespace pt {

@@ -7,5 +7,6 @@ namespace pb {
 MPGOS_PB_FACTORY(ring_duffing, ::mpgos::systems::RingDuffing)
 MPGOS_PB_FACTORY(ring_dw, ::mpgos::systems::RingDW)
 MPGOS_PB_FACTORY(pb_linear, ::mpgos::systems::PBLinear)
+MPGOS_PB_FACTORY(ring_duffing_q, ::mpgos::systems::RingDuffingQ)
 } // namespace pb
 } // namespace mpgos

@@ -53,6 +53,48 @@ struct RingDuffing : PBNoCallbacks {
     }
 };
 
+// RingDuffing with a quadrature state q' = x^2 per unit (UD = 3), for losses
+// of the form mean over a window of x^2; with the vector-Jacobian product
+// that the discrete adjoint needs (gradients with respect to the state, the
+// unit parameter k and the system parameter B).
+struct RingDuffingQ : PBNoCallbacks {
+    static constexpr const char* name = "ring_duffing_q";
+    static constexpr int UD = 3, NC = 1, NUP = 1, NSP = 1, NGP = 0, NiGP = 0, NUA = 0, NiUA = 0, NSA = 0, NiSA = 0, NE = 0;
+
+    template <class P>
+    MPGOS_HD static void OdeFunction(int, int, P* F, P* X, P T, P* uPAR, P* sPAR, P*, int*, P*, int*, P*, int*, P* CPT,
+                                     P* CPF)
+    {
+        using mpgos::math::cos;
+        using mpgos::math::fma;
+        const P x = X[0], v = X[1];
+        F[0] = v;
+        F[1] = fma(-uPAR[0], v, sPAR[0] * cos(T)) - fma(x * x, x, x);
+        F[2] = x * x;
+        CPT[0] = x;
+        CPF[0] = P(1.0);
+    }
+
+    template <class P>
+    MPGOS_HD static void OdeVJP(int, int, const P* kbF, const P* kbCPT, const P*, P* X, P T, P* uPAR, P*, P*, int*, P*, int*,
+                                P*, int*, P* gX, P* gU, P* gS)
+    {
+        using mpgos::math::cos;
+        const P x = X[0], v = X[1];
+        gX[0] = -(P(1) + P(3) * x * x) * kbF[1] + P(2) * x * kbF[2] + kbCPT[0];
+        gX[1] = kbF[0] - uPAR[0] * kbF[1];
+        gX[2] = P(0);
+        gU[0] = -v * kbF[1];
+        gS[0] = cos(T) * kbF[1];
+    }
+
+    template <class P>
+    MPGOS_HD static void Initialization(int, int uid, int&, P& T, P&, P* TD, P*, P*, P*, P*, int*, P*, int*, P*, int*)
+    {
+        if (uid == 0) T = TD[0];
+    }
+};
+
 // The reference's per-block tutorial system (double-well Duffing ring, two
 // couplings, unit events and user termination), same expressions and callback
 // behaviour as RingOfDuffingOscillators_SystemDefinition.cuh:7-104:
@@ -142,6 +184,16 @@ struct PBLinear : PBNoCallbacks {
         F[0] = uPAR[0] * X[0];
         CPT[0] = X[0];
         CPF[0] = P(1.0);
+    }
+
+    // vector-Jacobian product: adjoints kbF of F, kbCPT of CPT and kbCPF of CPF
+    // to adjoints of X, of the unit parameters and of the system parameters
+    template <class P>
+    MPGOS_HD static void OdeVJP(int, int, const P* kbF, const P* kbCPT, const P*, P* X, P, P* uPAR, P*, P*, int*, P*, int*,
+                                P*, int*, P* gX, P* gU, P*)
+    {
+        gX[0] = uPAR[0] * kbF[0] + kbCPT[0];
+        gU[0] = X[0] * kbF[0];
     }
 
     template <class P>

@@ -60,6 +60,7 @@ class PBOption(enum.IntEnum):
     SharedGlobalVariables = 12
     SharedCouplingMatrices = 13
     CouplingKernel = 14          # new: 0 automatic, 1 lane per pair, 2 dense contraction (MFMA)
+    MfmaUnitsPerLane = 15        # new: 4 (default) or 2 units per lane in the dense-contraction kernel
 
 
 class _PBDims(ctypes.Structure):

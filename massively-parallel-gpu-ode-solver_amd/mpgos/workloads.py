@@ -239,6 +239,7 @@ PB_SYSTEM_DIMS = {
     "ring_duffing": (2, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0),
     "ring_dw":      (2, 2, 3, 0, 1, 0, 3, 1, 2, 1, 1),
     "pb_linear":    (1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0),
+    "ring_duffing_q": (3, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0),
 }
 
 
@@ -287,13 +288,13 @@ def new_pb_spec(system, algorithm, ns, ups, cbw=0, cci=0, ndo=0):
     }
 
 
-def config4_ring(ns=4096, ups=256, periods=4, algorithm=Algorithms.RKCK45):
+def config4_ring(ns=4096, ups=256, periods=4, algorithm=Algorithms.RKCK45, system="ring_duffing"):
     """Per-block ring of U forced Duffing units with a dense coupling matrix on
     the velocity equation: C_uv ~ N(0, 0.05/U) (variance), zero diagonal, one
     matrix shared by all systems; per-unit k = 0.2, per-system B ~ U[0.5, 5];
     unit initial states (x, v) ~ U[-1, 1]^2; RKCK45 at rtol = atol = 1e-9 over
     `periods` forcing periods (4 by default)."""
-    s = new_pb_spec("ring_duffing", algorithm, ns, ups)
+    s = new_pb_spec(system, algorithm, ns, ups)
     rs = Stream(SEED, 4)
     C = rs.normal(ups * ups, np.sqrt(0.05 / ups)).reshape(ups, ups)
     np.fill_diagonal(C, 0.0)
@@ -301,7 +302,7 @@ def config4_ring(ns=4096, ups=256, periods=4, algorithm=Algorithms.RKCK45):
     s["cMat"] = store_coupling(C, 0, 0)
     s["uPAR"][0] = 0.2
     s["sPAR"][0] = rs.uniform(ns, 0.5, 5.0)
-    s["X"][:] = rs.uniform(2 * ns * ups, -1.0, 1.0).reshape(2, ns * ups)
+    s["X"][:2] = rs.uniform(2 * ns * ups, -1.0, 1.0).reshape(2, ns * ups)
     s["TD"][1] = periods * (2.0 * PI)
     s["strength"][0] = 1.0
     s["cIndex"][0] = 1

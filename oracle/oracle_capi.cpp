@@ -10,6 +10,7 @@
 #include "dual.h"
 #include "flops.h"
 #include "pb_oracle.h"
+#include "pb_sens.h"
 #include "pt_oracle.h"
 #include "systems/affine2.h"
 #include "systems/duffing.h"
@@ -160,11 +161,33 @@ int oracle_pb_solve(const char* system, oracle::PBJob* job, double* seconds)
     if (std::strcmp(system, "ring_duffing") == 0) fn = oracle::run_pb<mpgos::systems::RingDuffing>;
     else if (std::strcmp(system, "ring_dw") == 0) fn = oracle::run_pb<mpgos::systems::RingDW>;
     else if (std::strcmp(system, "pb_linear") == 0) fn = oracle::run_pb<mpgos::systems::PBLinear>;
+    else if (std::strcmp(system, "ring_duffing_q") == 0) fn = oracle::run_pb<mpgos::systems::RingDuffingQ>;
     if (!fn) return -4;
     const auto t0 = std::chrono::steady_clock::now();
     fn(*job);
     const auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return 0;
+}
+
+// Frozen-step forward sensitivities of every system (pb_sens.h):
+// jac[((s * N + i) * UD + c) * UPS + u], N = NSP + UD + NUP of the system.
+int oracle_pb_sensitivity(const char* system, oracle::PBJob* job, double* jac, int* ndir)
+{
+    if (!system || !job || !jac) return -4;
+    int n = -1;
+    auto go = [&](auto tag) {
+        using Sys = decltype(tag);
+        using S = oracle::PBSens<Sys>;
+        n = S::N;
+        const size_t per = (size_t)S::N * Sys::UD * job->ups;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(job->nthreads > 0 ? job->nthreads : omp_get_max_threads())
+        for (int sid = 0; sid < job->n_active; sid++) S::run(*job, sid, jac + per * sid);
+    };
+    if (std::strcmp(system, "ring_duffing_q") == 0) go(mpgos::systems::RingDuffingQ{});
+    else if (std::strcmp(system, "pb_linear") == 0) go(mpgos::systems::PBLinear{});
+    else return -4;
+    if (ndir) *ndir = n;
     return 0;
 }
 

@@ -53,6 +53,7 @@ struct PBSystem {
     int doIdx, skipped, stop, userStop;
     long long accepted = 0, rejected = 0;
     int flags = 0;
+    std::vector<double>* trace = nullptr;   // when set: (t, h) of every accepted step
     // unit scope
     std::vector<double> x, up, ua, ef, cpf, cpt;
     std::vector<int> uai;
@@ -240,6 +241,7 @@ struct PBSystem {
                 if (stop == 0) rejected++;
                 continue;
             }
+            if (trace) { trace->push_back(t); trace->push_back(dt); }
             x = next;
             t += dt;
             for (int u = 0; u < U; u++) Sys::template ActionAfterSuccessfulTimeStep<double>(sid, u, userStop, MPGOS_PBO_USER(u, t, dt));

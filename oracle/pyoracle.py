@@ -198,14 +198,9 @@ class PBJob(ctypes.Structure):
     ]
 
 
-def run_pb(system, spec, nthreads=0):
-    """Integrate a per-block spec (see mpgos.workloads.new_pb_spec) with the
-    oracle; returns the output arrays (copies) plus 'seconds'."""
-    L = lib()
-    if not hasattr(L, "_pb_set"):
-        L.oracle_pb_solve.restype = ctypes.c_int
-        L.oracle_pb_solve.argtypes = [ctypes.c_char_p, ctypes.POINTER(PBJob), ctypes.POINTER(ctypes.c_double)]
-        L._pb_set = True
+def _pb_job(spec, out, nthreads=0):
+    """PBJob over copies of the spec's arrays; `out` receives the arrays (the
+    outputs are written there)."""
     ns, ups, ud = spec["ns"], spec["ups"], spec["ud"]
     nu = ns * ups
     ndo = spec.get("ndo", 0)
@@ -215,7 +210,7 @@ def run_pb(system, spec, nthreads=0):
     def cp(name, dtype):
         return np.ascontiguousarray(np.array(spec[name], dtype=dtype, copy=True))
 
-    out = {k: cp(k, np.float64) for k in ("X", "uPAR", "uACC", "TD", "T", "sPAR", "sACC", "strength", "gPAR", "cMat")}
+    out.update({k: cp(k, np.float64) for k in ("X", "uPAR", "uACC", "TD", "T", "sPAR", "sACC", "strength", "gPAR", "cMat")})
     out.update({k: cp(k, np.int32) for k in ("uACCi", "sACCi", "gPARi", "cIndex", "DOidx")})
     out["DOt"] = np.zeros((ndo, ns))
     out["DOx"] = np.zeros((ndo, ud, nu))
@@ -223,10 +218,10 @@ def run_pb(system, spec, nthreads=0):
     out["nrej"] = np.zeros(ns, np.int64)
     out["status"] = np.zeros(ns, np.int32)
     ne = spec["ne"]
-    rtol = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("rtol", 1e-8), np.float64), (ud,)))
-    atol = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("atol", 1e-8), np.float64), (ud,)))
-    etol = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("etol", 1e-6), np.float64), (max(ne, 1),)))
-    edir = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("edir", 0), np.int32), (max(ne, 1),)))
+    out["_rtol"] = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("rtol", 1e-8), np.float64), (ud,)))
+    out["_atol"] = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("atol", 1e-8), np.float64), (ud,)))
+    out["_etol"] = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("etol", 1e-6), np.float64), (max(ne, 1),)))
+    out["_edir"] = np.ascontiguousarray(np.broadcast_to(np.asarray(spec.get("edir", 0), np.int32), (max(ne, 1),)))
     j = PBJob()
     j.ns, j.ups, j.n_active, j.ndo = ns, ups, int(spec.get("n_active", ns)), ndo
     j.algorithm, j.cbw, j.cci = int(spec["algorithm"]), int(spec["cbw"]), int(spec["cci"])
@@ -234,15 +229,52 @@ def run_pb(system, spec, nthreads=0):
     for k in ("dt0", "dtmax", "dtmin", "grow", "shrink", "do_min_dt"):
         setattr(j, k, float(o[k]))
     j.do_freq = int(o["do_freq"])
-    j.rtol, j.atol, j.etol, j.edir = _ptr(rtol, _dp), _ptr(atol, _dp), _ptr(etol, _dp), _ptr(edir, _ip)
+    j.rtol, j.atol = _ptr(out["_rtol"], _dp), _ptr(out["_atol"], _dp)
+    j.etol, j.edir = _ptr(out["_etol"], _dp), _ptr(out["_edir"], _ip)
     for k in ("X", "uPAR", "uACC", "TD", "T", "sPAR", "sACC", "strength", "DOt", "DOx", "gPAR", "cMat"):
         setattr(j, k, _ptr(out[k], _dp))
     for k in ("uACCi", "sACCi", "DOidx", "gPARi", "cIndex", "status"):
         setattr(j, k, _ptr(out[k], _ip))
     j.nacc, j.nrej = _ptr(out["nacc"], _lp), _ptr(out["nrej"], _lp)
     j.nthreads = int(nthreads)
+    return j
+
+
+def run_pb(system, spec, nthreads=0):
+    """Integrate a per-block spec (see mpgos.workloads.new_pb_spec) with the
+    oracle; returns the output arrays (copies) plus 'seconds'."""
+    L = lib()
+    if not hasattr(L, "_pb_set"):
+        L.oracle_pb_solve.restype = ctypes.c_int
+        L.oracle_pb_solve.argtypes = [ctypes.c_char_p, ctypes.POINTER(PBJob), ctypes.POINTER(ctypes.c_double)]
+        L._pb_set = True
+    out = {}
+    j = _pb_job(spec, out, nthreads)
     sec = ctypes.c_double()
     if L.oracle_pb_solve(system.encode(), ctypes.byref(j), ctypes.byref(sec)) != 0:
         raise RuntimeError(f"oracle per-block failed for {system!r}")
+    for k in ("_rtol", "_atol", "_etol", "_edir"):
+        out.pop(k)
     out["seconds"] = sec.value
     return out
+
+
+def sensitivity_pb(system, spec, nthreads=0):
+    """Frozen-step forward sensitivities of a per-block spec (pb_sens.h):
+    array [ns, N, UD, UPS] with directions N = (system parameters, initial
+    state of unit 0, unit parameters of unit 0)."""
+    L = lib()
+    if not hasattr(L, "_pbsens_set"):
+        L.oracle_pb_sensitivity.restype = ctypes.c_int
+        L.oracle_pb_sensitivity.argtypes = [ctypes.c_char_p, ctypes.POINTER(PBJob), _dp, ctypes.POINTER(ctypes.c_int)]
+        L._pbsens_set = True
+    ns, ups, ud = spec["ns"], spec["ups"], spec["ud"]
+    n = spec["nsp"] + ud + spec["nup"]
+    jac = np.zeros((ns, n, ud, ups))
+    keep = {}
+    j = _pb_job(spec, keep, nthreads)
+    nd = ctypes.c_int()
+    if L.oracle_pb_sensitivity(system.encode(), ctypes.byref(j), _ptr(jac, _dp), ctypes.byref(nd)) != 0:
+        raise RuntimeError(f"oracle per-block sensitivity failed for {system!r}")
+    assert nd.value == n
+    return jac

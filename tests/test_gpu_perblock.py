@@ -178,3 +178,43 @@ def test_second_solve_continues(oracle):
     spec2["TD"] = np.stack([spec["TD"][1], 2.0 * spec["TD"][1]])
     ref2 = pyoracle.run_pb("ring_duffing", spec2)
     assert_pb_bitwise(second, ref2, "second")
+
+
+def test_dropin_cpp_driver_matches_oracle(oracle, tmp_path):
+    """The reference tutorial's driver, compiled against the drop-in
+    CoupledSystems_PerBlock_Interface.cuh, reproduces the oracle bit for bit;
+    its Print files follow the reference's layout (unit scope padded to TPB)."""
+    import os
+    import subprocess
+    import pyoracle
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "ring_dropin")
+    assert os.path.exists(exe), "build() compiles tests/cpp/ring_dropin"
+    out = tmp_path / "out.bin"
+    print("ring_dropin driver", flush=True)
+    r = subprocess.run([exe, str(out)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ns, ups = 25, 14
+    nu = ns * ups
+    v = np.fromfile(out, dtype=np.float64)
+    parts = {}
+    o = 0
+    for name, n in (("X", 2 * nu), ("T", ns), ("uACC", 3 * nu), ("uACCi", nu), ("sACC", 2 * ns), ("sACCi", ns), ("DOidx", ns)):
+        parts[name] = v[o:o + n]
+        o += n
+    assert o == v.size
+    ref = pyoracle.run_pb("ring_dw", W.tutorial_ring_dw())
+    got = {"X": parts["X"].reshape(2, nu), "T": parts["T"], "uACC": parts["uACC"].reshape(3, nu),
+           "uACCi": parts["uACCi"].reshape(1, nu).astype(np.int32), "sACC": parts["sACC"].reshape(2, ns),
+           "sACCi": parts["sACCi"].reshape(1, ns).astype(np.int32), "DOidx": parts["DOidx"].astype(np.int32)}
+    for k, a in got.items():
+        b = np.asarray(ref[k])
+        same = (a.view(np.uint64) == b.view(np.uint64)) if a.dtype == np.float64 else (a == b)
+        assert same.all(), k
+    rows = (tmp_path / "ActualStateInSolverObject.txt").read_text().splitlines()
+    assert len(rows) == 9 * 64 and sum("PADDING" in r for r in rows) == 9 * 64 - nu
+    first = [x for x in rows[0].split(",") if x.strip()]
+    assert float(first[0]) == pytest.approx(ref["X"][0, 0], rel=1e-9)
+    dense = (tmp_path / "DenseOutput_0.txt").read_text().splitlines()
+    series = dense[dense.index("Time series:") + 2:]
+    assert len(series) == ref["DOidx"][0]
+    assert float(series[-1].split(",")[0]) == pytest.approx(ref["T"][0], rel=1e-9)

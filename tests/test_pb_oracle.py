@@ -275,3 +275,68 @@ def test_host_reference_values(oracle):
     o = pyoracle.run_pb("ring_dw", s)
     assert o["T"][0] == pytest.approx(pb["t_end"], abs=1e-5)
     assert np.abs(o["X"][0] - pb["x0_end"]).max() < 1e-13
+
+
+def test_pb_sensitivity_linear_closed_form(oracle):
+    """Frozen-step sensitivities of the linear ring under fixed-step RK4:
+    d x_N / d x_0(unit 0) is column 0 of R(hA)^N, and d x_N / d a_0 matches
+    the complex-step derivative of the closed form."""
+    import pyoracle
+    n, h = 32, 1.0 / 32.0
+    s, M = _linear_spec((0, 0), ns=2, ups=6, n=n, h=h)
+    jac = pyoracle.sensitivity_pb("pb_linear", s)           # [ns, N=2, UD=1, U]
+    ups = s["ups"]
+    for sid in range(2):
+        sl = slice(sid * ups, (sid + 1) * ups)
+        a = s["uPAR"][0, sl].copy()
+
+        def end_state(a_vec):
+            A = np.diag(a_vec) + s["strength"][0, sid] * M
+            R = _rk4_poly(h * A)
+            return np.linalg.matrix_power(R, n + 1 - 1) @ s["X"][0, sl], np.linalg.matrix_power(R, n)
+
+        _, Rn = end_state(a)
+        assert np.allclose(jac[sid, 0, 0], Rn[:, 0], rtol=1e-11, atol=1e-13)
+        # complex step: d/da_0 of the closed form to machine precision
+        ac = a.astype(complex)
+        ac[0] += 1e-30j
+        cs = end_state(ac)[0].imag / 1e-30
+        assert np.allclose(jac[sid, 1, 0], cs, rtol=1e-10, atol=1e-13)
+
+
+def test_pb_sensitivity_ring_vs_finite_differences(oracle):
+    """The dense Duffing ring with a quadrature state under RK4 (fixed steps,
+    so frozen and free step sequences agree): sensitivities with respect to
+    B, the initial state of unit 0 and k of unit 0 against central differences
+    (h = 1e-6) of the oracle itself."""
+    import pyoracle
+    s = W.config4_ring(ns=2, ups=16, periods=1, algorithm=Algorithms.RK4, system="ring_duffing_q")
+    s["options"]["dt0"] = 2.0 * W.PI / 200.0
+    jac = pyoracle.sensitivity_pb("ring_duffing_q", s)      # [2, 5, 3, 16]
+    e = 1e-6
+
+    def fd(mod):
+        sp, sm = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in s.items()}, \
+                 {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in s.items()}
+        mod(sp, +e)
+        mod(sm, -e)
+        xp = pyoracle.run_pb("ring_duffing_q", sp)["X"]
+        xm = pyoracle.run_pb("ring_duffing_q", sm)["X"]
+        return (xp - xm) / (2 * e)                          # [3, ns*U]
+
+    def dB(t, d):
+        t["sPAR"][0, :] += d
+    def dx0(c):
+        def f(t, d):
+            t["X"][c, [0, 16]] += d
+        return f
+    def dk(t, d):
+        t["uPAR"][0, [0, 16]] += d
+    cases = [(0, dB)] + [(1 + c, dx0(c)) for c in range(3)] + [(4, dk)]
+    for i, mod in cases:
+        ref = fd(mod)
+        for sid in range(2):
+            got = jac[sid, i]
+            want = ref[:, sid * 16:(sid + 1) * 16]
+            scale = max(1.0, np.abs(want).max())
+            assert np.abs(got - want).max() < 2e-7 * scale, (i, sid)
