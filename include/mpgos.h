@@ -255,6 +255,9 @@ enum {
     MPGOS_PB_ACCEPTED_STEPS,   /* int64 [NS] */
     MPGOS_PB_REJECTED_STEPS,   /* int64 [NS] */
     MPGOS_PB_STATUS,           /* int32 [NS]: MPGOS_STATUS_* bits */
+    MPGOS_PB_ADJOINT_STATE,    /* FP64 [UD][NS*UPS]: dL/dx(end) in, dL/dx(start) out (mpgos_pb_adjoint_reverse) */
+    MPGOS_PB_UNIT_PARAMETER_GRADIENT,    /* FP64 [NUP][NS*UPS]: dL/d(unit parameters) */
+    MPGOS_PB_SYSTEM_PARAMETER_GRADIENT,  /* FP64 [NSP][NS]: dL/d(system parameters) */
     MPGOS_PB_NUM_VARIABLES
 };
 
@@ -327,6 +330,22 @@ int mpgos_pb_synchronise_device(mpgos_solver* h);
 int mpgos_pb_insert_synchronisation_point(mpgos_solver* h);
 int mpgos_pb_synchronise_solver(mpgos_solver* h);
 int mpgos_pb_last_solve_ms(const mpgos_solver* h, double* ms);
+
+/* Discrete adjoint of the per-block integration (new; no reference counterpart).
+ * FP64 systems that provide OdeVJP.  The accepted step sizes are taken as given
+ * (frozen), the controller and event step control are not differentiated.
+ * mpgos_pb_adjoint_record: a Solve() that also stores the unit states every
+ *   `checkpoint_interval` accepted steps (at most `checkpoint_capacity` per
+ *   system) and (t, h) of every accepted step (at most `step_capacity` per
+ *   system); a system that needs more gets MPGOS_STATUS_CHECKPOINT_OVERFLOW and
+ *   NaN gradients.
+ * mpgos_pb_adjoint_reverse: with dL/dx(end) in MPGOS_PB_ADJOINT_STATE (device
+ *   buffer), computes dL/dx(start) (same buffer), dL/d(unit parameters) and
+ *   dL/d(system parameters) by recomputing each checkpoint segment and
+ *   back-propagating through the stages and the coupling terms (transposed
+ *   coupling product).  Asynchronous on the solver's stream. */
+int mpgos_pb_adjoint_record(mpgos_solver* h, int checkpoint_interval, int checkpoint_capacity, int step_capacity);
+int mpgos_pb_adjoint_reverse(mpgos_solver* h);
 void* mpgos_pb_stream(mpgos_solver* h);
 const char* mpgos_pb_list_systems(void);
 

@@ -256,6 +256,12 @@ int mpgos_pb_last_solve_ms(const mpgos_solver* h, double* ms)
     if (!ms) { set_error("null pointer"); return MPGOS_E_ARG; }
     return h->pb->last_solve_ms(ms);
 }
+int mpgos_pb_adjoint_record(mpgos_solver* h, int checkpoint_interval, int checkpoint_capacity, int step_capacity)
+{
+    MPGOS_CHECK_PB(h);
+    return h->pb->adjoint_record(checkpoint_interval, checkpoint_capacity, step_capacity);
+}
+int mpgos_pb_adjoint_reverse(mpgos_solver* h) { MPGOS_CHECK_PB(h); return h->pb->adjoint_reverse(); }
 void* mpgos_pb_stream(mpgos_solver* h)
 {
     if (!h || !h->pb) { set_error("null or non-per-block solver handle"); return nullptr; }

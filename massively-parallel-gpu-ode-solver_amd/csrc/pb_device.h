@@ -77,6 +77,11 @@ struct Args {
     P rtol[UD], atol[UD];
     P etol[Pos<NE>::value];
     int edir[Pos<NE>::value];
+    // record pass of the discrete adjoint (unused otherwise): unit states every
+    // recK accepted steps, recX[(k * UD + c) * NU + unit], k < recCkCap; time
+    // and size of every accepted step, recT/recH[n * NS + system], n < recStepCap
+    P* recX; P* recT; P* recH;
+    int recK, recCkCap, recStepCap;
 };
 
 // System scope of one system (LDS).
@@ -249,7 +254,7 @@ __device__ __forceinline__ void mfma_contract(const double* M, const double* cpt
 // u0 + p * (UPS / PAIRS), u0 = t % (UPS / PAIRS), so the system-scope scalars
 // (time, step) are shared by the lane's pairs and global accesses to unit-scope
 // arrays stay coalesced.
-template <class Sys, class P, int ALG, int PAIRS, int CPL>
+template <class Sys, class P, int ALG, int PAIRS, int CPL, bool REC = false>
 __device__ void pb_body(const Args<Sys, P>& a)
 {
     constexpr int UD = Sys::UD, NC = Sys::NC, NUP = Sys::NUP, NSP = Sys::NSP, NGP = Sys::NGP, NiGP = Sys::NiGP;
@@ -376,6 +381,13 @@ __device__ void pb_body(const Args<Sys, P>& a)
             my->doT = dmin<P>(my->t + a.doMinDt, my->TD[1]);
         }
         __syncthreads();
+    }
+
+    if (REC && active) {        // checkpoint 0: the initial unit states
+#pragma unroll
+        for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+            for (int c = 0; c < UD; c++) a.recX[(size_t)c * NU + gu(p)] = X[p][c];
     }
 
     int buf = 0;
@@ -602,6 +614,24 @@ __device__ void pb_body(const Args<Sys, P>& a)
             for (int p = 0; p < PAIRS; p++)
 #pragma unroll
                 for (int c = 0; c < UD; c++) X[p][c] = next[p][c];
+            if (REC) {
+                const long long n = my->nacc;            // accepted steps before this one
+                if (leader) {
+                    if (n < a.recStepCap) { a.recT[(size_t)n * NS + gsys] = my->t; a.recH[(size_t)n * NS + gsys] = my->dt; }
+                    else my->status |= MPGOS_STATUS_CHECKPOINT_OVERFLOW;
+                }
+                if ((n + 1) % a.recK == 0) {
+                    const long long k = (n + 1) / a.recK;
+                    if (k < a.recCkCap) {
+#pragma unroll
+                        for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                            for (int c = 0; c < UD; c++) a.recX[((size_t)k * UD + c) * NU + gu(p)] = X[p][c];
+                    } else if (leader) {
+                        my->status |= MPGOS_STATUS_CHECKPOINT_OVERFLOW;
+                    }
+                }
+            }
         }
         __syncthreads();
         if (leader && upd == 1) my->t += my->dt;
@@ -722,6 +752,19 @@ template <class Sys, class P, int ALG, int MAXT>
 __global__ void __launch_bounds__(MAXT) pb_kernel(Args<Sys, P> a)
 {
     pb_body<Sys, P, ALG, 1, kLaneCoupling>(a);
+}
+
+// Record pass of the discrete adjoint (FP64): the forward integration plus
+// checkpoints (lane-per-pair or dense-contraction form).
+template <class Sys, int ALG, int MAXT>
+__global__ void __launch_bounds__(MAXT) pb_record_kernel(Args<Sys, double> a)
+{
+    pb_body<Sys, double, ALG, 1, kLaneCoupling, true>(a);
+}
+template <class Sys, int ALG>
+__global__ void __launch_bounds__(512) pb_record_mfma_kernel(Args<Sys, double> a)
+{
+    pb_body<Sys, double, ALG, 4, kMfmaCoupling, true>(a);
 }
 
 // Dense-contraction form: SPB = 8, PAIRS units per lane, 8 * UPS / PAIRS

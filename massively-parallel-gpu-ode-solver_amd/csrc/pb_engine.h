@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "mpgos.h"
+#include "pb_adjoint.h"
 #include "pb_device.h"
 #include "pt_engine.h"   // Buffer, set_error, MPGOS_HIP_TRY
 
@@ -43,7 +44,14 @@ public:
     virtual int synchronise_solver() = 0;
     virtual int last_solve_ms(double* ms) const = 0;
     virtual void* stream() = 0;
+    virtual int adjoint_record(int K, int checkpoint_capacity, int step_capacity) = 0;
+    virtual int adjoint_reverse() = 0;
 };
+
+// systems that provide OdeVJP (needed by the discrete adjoint)
+template <class Sys, class = void> struct HasVjp : std::false_type {};
+template <class Sys>
+struct HasVjp<Sys, decltype((void)&Sys::template OdeVJP<double>)> : std::true_type {};
 
 template <class Sys, class P>
 class Solver : public SolverBase {
@@ -70,9 +78,19 @@ public:
     Buffer<long long> nacc, nrej;
     std::vector<P> cMatOriginal;   // the full matrices as set (GetHost), NC * UPS * UPS
 
+    // discrete adjoint (FP64 systems with OdeVJP)
+    static constexpr bool kAdjoint = std::is_same<P, double>::value && HasVjp<Sys>::value;
+    Buffer<P> lamB, gUB, gSB;           // dL/dx, dL/d(unit parameters), dL/d(system parameters)
+    double* recX = nullptr; double* recT = nullptr; double* recH = nullptr; double* seg = nullptr; double* cMatT = nullptr;
+    size_t recXn = 0, recTn = 0, recHn = 0, segn = 0, cMatTn = 0;
+    int adjK = 0, adjCap = 0, adjStepCap = 0;
+
     ~Solver() override
     {
         if (strm) (void)hipStreamSynchronize(strm);
+        lamB.release(); gUB.release(); gSB.release();
+        for (double* q : {recX, recT, recH, seg, cMatT})
+            if (q) (void)hipFree(q);
         for (auto* b : {&X, &uPAR, &uACC, &TD, &T, &sPAR, &sACC, &strength, &DOt, &DOx, &gPAR, &cMat}) b->release();
         for (auto* b : {&uACCi, &sACCi, &DOidx, &gPARi, &cIndex, &status}) b->release();
         nacc.release(); nrej.release();
@@ -113,6 +131,8 @@ public:
             (rc = DOx.alloc((size_t)NDO * UD * NU)) || (rc = gPAR.alloc(NGP)) || (rc = gPARi.alloc(NiGP)) ||
             (rc = cIndex.alloc(NC)) || (rc = cMat.alloc((size_t)NC * cMatSize)) || (rc = status.alloc(NS)) ||
             (rc = nacc.alloc(NS)) || (rc = nrej.alloc(NS)))
+            return rc;
+        if (kAdjoint && ((rc = lamB.alloc(UD * NU)) || (rc = gUB.alloc(NUP * NU)) || (rc = gSB.alloc((size_t)NSP * NS))))
             return rc;
         cMatOriginal.assign((size_t)NC * UPS * UPS, P(0));
         rtol.assign(UD, 1e-8);
@@ -313,6 +333,9 @@ public:
         case MPGOS_PB_ACCEPTED_STEPS: return ret(nacc, host, nbytes);
         case MPGOS_PB_REJECTED_STEPS: return ret(nrej, host, nbytes);
         case MPGOS_PB_STATUS: return ret(status, host, nbytes);
+        case MPGOS_PB_ADJOINT_STATE: return ret(lamB, host, nbytes);
+        case MPGOS_PB_UNIT_PARAMETER_GRADIENT: return ret(gUB, host, nbytes);
+        case MPGOS_PB_SYSTEM_PARAMETER_GRADIENT: return ret(gSB, host, nbytes);
         default:
             set_error("buffer: variable has no single buffer");
             if (nbytes) *nbytes = 0;
@@ -366,6 +389,9 @@ public:
             case MPGOS_PB_DENSE_INDEX: return copy(DOidx, to_device);
             case MPGOS_PB_DENSE_TIME: return copy(DOt, to_device);
             case MPGOS_PB_DENSE_STATE: return copy(DOx, to_device);
+            case MPGOS_PB_ADJOINT_STATE: return copy(lamB, to_device);
+            case MPGOS_PB_UNIT_PARAMETER_GRADIENT: return copy(gUB, to_device);
+            case MPGOS_PB_SYSTEM_PARAMETER_GRADIENT: return copy(gSB, to_device);
             default: set_error("Synchronise: variable not applicable"); return MPGOS_E_ARG;
             }
         }
@@ -444,6 +470,123 @@ public:
         MPGOS_HIP_TRY(hipGetLastError());
         MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
         return MPGOS_OK;
+    }
+
+    static int grow_device(double*& p, size_t& have, size_t need, const char* what)
+    {
+        if (need <= have) return MPGOS_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr; have = 0;
+        if (hipMalloc((void**)&p, need * sizeof(double)) != hipSuccess) {
+            set_error(std::string("per-block adjoint: allocation of ") + what + " failed");
+            return MPGOS_E_MEMORY;
+        }
+        have = need;
+        return MPGOS_OK;
+    }
+
+    // Record pass: a Solve() that also keeps the unit states every K accepted
+    // steps (at most checkpoint_capacity per system) and (t, h) of every
+    // accepted step (at most step_capacity per system).
+    int adjoint_record(int K, int ckcap, int stepcap) override
+    {
+        if constexpr (!kAdjoint) {
+            set_error("per-block adjoint: needs FP64 and a system with OdeVJP");
+            return MPGOS_E_SYSTEM;
+        } else {
+            if (K < 1 || ckcap < 1 || stepcap < 1) { set_error("per-block adjoint: interval and capacities must be >= 1"); return MPGOS_E_ARG; }
+            MPGOS_HIP_TRY(hipSetDevice(device));
+            const size_t NU = (size_t)NS * UPS;
+            MPGOS_HIP_TRY(hipStreamSynchronize(strm));
+            int rc;
+            if ((rc = grow_device(recX, recXn, (size_t)ckcap * UD * NU, "checkpoints")) || (rc = grow_device(recT, recTn, (size_t)stepcap * NS, "step times")) ||
+                (rc = grow_device(recH, recHn, (size_t)stepcap * NS, "step sizes")))
+                return rc;
+            adjK = K; adjCap = ckcap; adjStepCap = stepcap;
+            Args<Sys, double> a = make_args();
+            a.recX = recX; a.recT = recT; a.recH = recH; a.recK = K; a.recCkCap = ckcap; a.recStepCap = stepcap;
+            const int cpl = coupling_form();
+            const int spb = cpl == kMfmaCoupling ? kMfmaSystems : SPB;
+            a.SPB = spb;
+            a.sharedMatrix = sharedMatrix && lds_bytes(cpl, spb, 1) <= ldsMax ? 1 : 0;
+            const size_t lds = lds_bytes(cpl, spb, a.sharedMatrix);
+            if (lds > ldsMax) { set_error("per-block: system scope and coupling terms exceed the LDS of a workgroup"); return MPGOS_E_ARG; }
+            const int threads = cpl == kMfmaCoupling ? kMfmaSystems * UPS / 4 : ((SPB * UPS + 63) / 64) * 64;
+            const int grid = NS / spb + (NS % spb == 0 ? 0 : 1);
+            MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+            if (cpl == kMfmaCoupling) {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_record_mfma_kernel<Sys, MPGOS_RK4>), dim3(grid), dim3(threads), lds, strm, a);
+                else hipLaunchKernelGGL((pb_record_mfma_kernel<Sys, MPGOS_RKCK45>), dim3(grid), dim3(threads), lds, strm, a);
+            } else if (threads <= 256) {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_record_kernel<Sys, MPGOS_RK4, 256>), dim3(grid), dim3(threads), lds, strm, a);
+                else hipLaunchKernelGGL((pb_record_kernel<Sys, MPGOS_RKCK45, 256>), dim3(grid), dim3(threads), lds, strm, a);
+            } else {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_record_kernel<Sys, MPGOS_RK4, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a);
+                else hipLaunchKernelGGL((pb_record_kernel<Sys, MPGOS_RKCK45, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a);
+            }
+            MPGOS_HIP_TRY(hipGetLastError());
+            MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
+            return MPGOS_OK;
+        }
+    }
+
+    // Reverse pass: AdjointState holds dL/dx(end) on entry (device buffer) and
+    // dL/dx(start) on exit; UnitParameterGradient / SystemParameterGradient
+    // receive the parameter gradients.  Uses the last record pass.
+    int adjoint_reverse() override
+    {
+        if constexpr (!kAdjoint) {
+            set_error("per-block adjoint: needs FP64 and a system with OdeVJP");
+            return MPGOS_E_SYSTEM;
+        } else {
+            if (adjK == 0) { set_error("per-block adjoint: no record pass yet"); return MPGOS_E_STATE; }
+            MPGOS_HIP_TRY(hipSetDevice(device));
+            const size_t NU = (size_t)NS * UPS;
+            MPGOS_HIP_TRY(hipStreamSynchronize(strm));
+            int rc;
+            if ((rc = grow_device(seg, segn, (size_t)adjK * UD * NU, "segment buffer"))) return rc;
+            // transposed coupling matrices in the stored form (from the host mirror)
+            std::vector<double> mt((size_t)NC * cMatSize, 0.0);
+            for (int i = 0; i < NC; i++)
+                for (int r = 0; r < UPS; r++)
+                    for (int c = 0; c < UPS; c++) {
+                        const long dst = stored_index(i, r, c), src = stored_index(i, c, r);
+                        if (dst >= 0 && src >= 0) mt[dst] = (double)cMat.h[src];
+                    }
+            if ((rc = grow_device(cMatT, cMatTn, (size_t)NC * cMatSize, "transposed coupling matrix"))) return rc;
+            MPGOS_HIP_TRY(hipMemcpy(cMatT, mt.data(), mt.size() * sizeof(double), hipMemcpyHostToDevice));
+            Args<Sys, double> a = make_args();
+            a.recX = recX; a.recT = recT; a.recH = recH; a.recK = adjK; a.recCkCap = adjCap; a.recStepCap = adjStepCap;
+            AdjArgs<Sys> A;
+            A.seg = seg; A.lam = lamB.d; A.gU = gUB.d; A.gS = gSB.d; A.cMatT = cMatT;
+            const size_t lds = reverse_layout<Sys>(SPB, UPS).total;
+            if (lds > ldsMax) { set_error("per-block adjoint: reverse-pass LDS exceeds a workgroup's"); return MPGOS_E_ARG; }
+            const int threads = ((SPB * UPS + 63) / 64) * 64;
+            const int grid = NS / SPB + (NS % SPB == 0 ? 0 : 1);
+            MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+            if (threads <= 256) {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_reverse_kernel<Sys, MPGOS_RK4, 256>), dim3(grid), dim3(threads), lds, strm, a, A);
+                else hipLaunchKernelGGL((pb_reverse_kernel<Sys, MPGOS_RKCK45, 256>), dim3(grid), dim3(threads), lds, strm, a, A);
+            } else {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_reverse_kernel<Sys, MPGOS_RK4, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a, A);
+                else hipLaunchKernelGGL((pb_reverse_kernel<Sys, MPGOS_RKCK45, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a, A);
+            }
+            MPGOS_HIP_TRY(hipGetLastError());
+            MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
+            return MPGOS_OK;
+        }
+    }
+
+    // index of matrix element (row, col) of coupling i in the stored form, -1 outside the band (PB-IF:994-1081)
+    long stored_index(int i, int row, int col) const
+    {
+        if (CCI == 0 && CBW == 0) return (long)row + (long)col * UPS + (long)i * UPS * UPS;
+        int mc = col - row + CBW;
+        if (mc >= UPS) mc -= UPS;
+        if (mc < 0) mc += UPS;
+        const int lim = CBW > 0 ? 2 * CBW + 1 : UPS;
+        if (mc >= lim) return -1;
+        return CCI == 0 ? (long)row + (long)mc * UPS + (long)i * UPS * lim : (long)mc + (long)i * lim;
     }
 
     int synchronise_device() override { MPGOS_HIP_TRY(hipSetDevice(device)); MPGOS_HIP_TRY(hipDeviceSynchronize()); return MPGOS_OK; }

@@ -41,6 +41,9 @@ class PB(enum.IntEnum):
     AcceptedSteps = 19
     RejectedSteps = 20
     Status = 21
+    AdjointState = 22             # new: dL/dx (end in, start out) of the discrete adjoint
+    UnitParameterGradient = 23    # new: dL/d(unit parameters)
+    SystemParameterGradient = 24  # new: dL/d(system parameters)
 
 
 class PBOption(enum.IntEnum):
@@ -99,6 +102,8 @@ def _bind():
         "mpgos_pb_synchronise_solver": (c_int, [H]),
         "mpgos_pb_last_solve_ms": (c_int, [H, ctypes.POINTER(c_double)]),
         "mpgos_pb_stream": (ctypes.c_void_p, [H]),
+        "mpgos_pb_adjoint_record": (c_int, [H, c_int, c_int, c_int]),
+        "mpgos_pb_adjoint_reverse": (c_int, [H]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -187,6 +192,8 @@ class PBProblemSolver:
             PB.CouplingIndex: (np.int32, (d.nc,)), PB.DenseIndex: (np.int32, (d.ns,)), PB.DenseTime: (fp, (d.ndo, d.ns)),
             PB.DenseState: (fp, (d.ndo, d.ud, nu)), PB.AcceptedSteps: (np.int64, (d.ns,)),
             PB.RejectedSteps: (np.int64, (d.ns,)), PB.Status: (np.int32, (d.ns,)),
+            PB.AdjointState: (np.float64, (d.ud, nu)), PB.UnitParameterGradient: (np.float64, (d.nup, nu)),
+            PB.SystemParameterGradient: (np.float64, (d.nsp, d.ns)),
         }
         if var == PB.CouplingMatrix:
             n = ctypes.c_size_t()
@@ -233,6 +240,32 @@ class PBProblemSolver:
         n = ctypes.c_size_t()
         p = _bind().mpgos_pb_device_buffer(self._h, int(PB(var)), ctypes.byref(n))
         return p, n.value
+
+    # ---------------------------------------------------------------- discrete adjoint
+    def adjoint_record(self, checkpoint_interval, checkpoint_capacity, step_capacity):
+        """Solve() that also keeps the unit states every `checkpoint_interval`
+        accepted steps and (t, h) of every accepted step."""
+        check(_bind().mpgos_pb_adjoint_record(self._h, int(checkpoint_interval), int(checkpoint_capacity),
+                                              int(step_capacity)))
+
+    def adjoint_reverse(self):
+        """Reverse sweep: AdjointState holds dL/dx(end) on entry (device buffer),
+        dL/dx(start) on exit; the parameter gradients are filled."""
+        check(_bind().mpgos_pb_adjoint_reverse(self._h))
+
+    def adjoint(self, lam_end, checkpoint_interval, checkpoint_capacity, step_capacity):
+        """Record, upload dL/dx(end) = lam_end (UD x NS*UPS), reverse, read back.
+        Returns (dL/dx(start), dL/d(unit parameters), dL/d(system parameters))."""
+        self.adjoint_record(checkpoint_interval, checkpoint_capacity, step_capacity)
+        self.host_array(PB.AdjointState)[...] = lam_end
+        self.SynchroniseFromHostToDevice(PB.AdjointState)
+        self.adjoint_reverse()
+        for v in (PB.AdjointState, PB.UnitParameterGradient, PB.SystemParameterGradient):
+            self.SynchroniseFromDeviceToHost(v)
+        self.InsertSynchronisationPoint()
+        self.SynchroniseSolver()
+        return tuple(self.host_array(v).copy() for v in (PB.AdjointState, PB.UnitParameterGradient,
+                                                          PB.SystemParameterGradient))
 
     def solve_blocking(self):
         self.Solve()

@@ -105,3 +105,66 @@ def _reload(solver, k, B, x0, v0, t0, t1, opts, algorithm):
 
 def duffing_period_loss(k, B, x0, v0, algorithm=Algorithms.RK4, periods=64, K=32, opts=None):
     return DuffingPeriodLoss.apply(k, B, x0, v0, algorithm, periods, K, opts or {})
+
+
+class RingPeriodLoss(torch.autograd.Function):
+    """Per-system loss of a per-block ring with quadrature state
+    (ring_duffing_q): L_s = mean over units of the mean of x^2 over the last
+    forcing period, as two segments [0, H - 2pi] and [H - 2pi, H] with q reset
+    to 0.  Input: B (one per system).  backward() runs the per-block adjoint of
+    segment 2 (seed dL/dq = 1/(2pi U)), cuts the q component, then segment 1,
+    and returns dL/dB summed over both segments."""
+
+    @staticmethod
+    def forward(ctx, B, spec, K):
+        from mpgos.perblock import PB, PBProblemSolver
+        ns, ups = spec["ns"], spec["ups"]
+        H = spec["TD"][1, 0]
+        b = B.detach().cpu().numpy().astype(np.float64)
+
+        def run(x0, t0, t1):
+            sp = dict(spec)
+            sp["sPAR"] = b.reshape(1, -1)
+            sp["X"] = x0.copy()
+            sp["X"][2] = 0.0
+            sp["TD"] = np.stack([np.full(ns, t0), np.full(ns, t1)])
+            s = PBProblemSolver("ring_duffing_q", ns, ups, spec["algorithm"])
+            W.load_pb(s, sp)
+            s.solve_blocking()                      # sizes the record buffers
+            n = int(s.host_array(PB.AcceptedSteps).max())
+            W.load_pb(s, sp)
+            s.adjoint_record(K, n // K + 2, n + 2)
+            s.SynchroniseFromDeviceToHost(PB.All)
+            s.InsertSynchronisationPoint()
+            s.SynchroniseSolver()
+            return s, s.host_array(PB.ActualState).copy()
+        s1, xm = run(spec["X"], 0.0, H - TWO_PI)
+        s2, xe = run(xm, H - TWO_PI, H)
+        ctx.solvers = (s1, s2)
+        ctx.shape = (ns, ups)
+        return torch.from_numpy(xe[2].reshape(ns, ups).mean(axis=1) / TWO_PI)
+
+    @staticmethod
+    def backward(ctx, g):
+        from mpgos.perblock import PB
+        s1, s2 = ctx.solvers
+        ns, ups = ctx.shape
+        g = g.detach().cpu().numpy().astype(np.float64)
+
+        def reverse(s, lam):
+            s.host_array(PB.AdjointState)[...] = lam
+            s.SynchroniseFromHostToDevice(PB.AdjointState)
+            s.adjoint_reverse()
+            for v in (PB.AdjointState, PB.SystemParameterGradient):
+                s.SynchroniseFromDeviceToHost(v)
+            s.InsertSynchronisationPoint()
+            s.SynchroniseSolver()
+            return s.host_array(PB.AdjointState).copy(), s.host_array(PB.SystemParameterGradient).copy()
+        lam = np.zeros((3, ns * ups))
+        lam[2] = np.repeat(g / (TWO_PI * ups), ups)
+        lam_mid, gB2 = reverse(s2, lam)
+        lam_mid[2] = 0.0                            # q restarts at 0 in segment 2
+        _, gB1 = reverse(s1, lam_mid)
+        s1.close()
+        s2.close()
+        return torch.from_numpy(gB1[0] + gB2[0]), None, None
