@@ -189,11 +189,17 @@ struct Unit {
         }
     }
 
+    // F[ci] = fma(sc, v, F[ci]) for a run-time component ci, written as a
+    // select per component so that F stays in registers (a conditional store
+    // per component is merged by the compiler into one dynamically indexed
+    // store, which moves F to scratch memory)
     static __device__ __forceinline__ void add(int ci, P sc, P v, P* F)
     {
 #pragma unroll
-        for (int c = 0; c < UD; c++)
-            if (c == ci) F[c] = math::fma(sc, v, F[c]);
+        for (int c = 0; c < UD; c++) {
+            const P f = math::fma(sc, v, F[c]);
+            F[c] = c == ci ? f : F[c];
+        }
     }
 };
 

@@ -33,6 +33,9 @@ def _adjoint(spec, w, K, kernel=None):
     s = _solver(spec, kernel)
     try:
         lam0, gU, gS = s.adjoint(w, K, n // K + 2, n + 2)
+        s.SynchroniseFromDeviceToHost(PB.All)      # the recorded forward result
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
         out = W.read_pb(s)
     finally:
         s.close()
@@ -88,6 +91,9 @@ def test_ring_adjoint_checkpoint_interval_and_overflow(oracle):
     s = _solver(spec)
     try:
         lam0, gU, gS = s.adjoint(w, 4, 2, n + 2)       # capacity far too small
+        s.SynchroniseFromDeviceToHost(PB.Status)
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
         st = s.host_array(PB.Status).copy()
     finally:
         s.close()
