@@ -13,8 +13,10 @@ Other workloads (--workload), each printed as the same JSON line:
                   dt = 2pi/1000 over 64 forcing periods (64001 steps each);
   duffing_uneven  131072 Duffing oscillators with forcing frequency
                   eps ~ 10^U[-3,0] and horizons of 16..4096 periods, RKCK45 at
-                  1e-9, event x' = 0 with up to 256 logged hits (one Solve is
-                  long: tens of seconds);
+                  1e-9, event x' = 0 with up to 256 logged hits; the longest
+                  instances take ~6e7 steps, so the Solve runs time-sliced
+                  (--slice-steps per launch, bit-identical to one Solve) and
+                  reports progress after each slice;
   duffing_adjoint 65536 Duffing oscillators with a quadrature state, RK4 at
                   dt = 2pi/500 (or --algorithm rkck45 at 1e-9): forward Solve
                   and discrete adjoint (record + reverse), reporting the
@@ -68,7 +70,23 @@ def bitwise(a, b, keys):
 
 
 # --------------------------------------------------------------------------- per-thread workloads
-def time_form(mpgos, spec, cfg, steps, warmup, precision=0):
+def sliced_solve(s, budget, label):
+    """Solve in slices of `budget` attempted steps per instance (bit-identical
+    to one Solve); returns the summed kernel time in ms.  Each slice reports
+    the instances still unfinished."""
+    total = 0.0
+    k = 0
+    t0 = time.perf_counter()
+    while True:
+        left = s.solve_slice(budget)
+        total += s.last_solve_ms()
+        k += 1
+        log(f"{label}: slice {k}: {left} instances unfinished, kernel {total / 1e3:.1f} s, wall {time.perf_counter() - t0:.1f} s")
+        if left == 0:
+            return total
+
+
+def time_form(mpgos, spec, cfg, steps, warmup, precision=0, slice_steps=0):
     """K timed steps (device-side input reset + Solve) of one launch configuration."""
     from mpgos import workloads as W
     from mpgos.tuning import DeviceSnapshot, apply
@@ -79,12 +97,20 @@ def time_form(mpgos, spec, cfg, steps, warmup, precision=0):
         snap = DeviceSnapshot(s)
         for _ in range(warmup):
             snap.restore()
-            s.Solve()
+            if slice_steps:
+                snap.sync()
+                sliced_solve(s, slice_steps, f"warmup {cfg}")
+            else:
+                s.Solve()
         snap.sync()
         kernel_ms = []
         t0 = time.perf_counter()
         for i in range(steps):
             snap.restore()
+            if slice_steps:
+                snap.sync()
+                kernel_ms.append(sliced_solve(s, slice_steps, f"{cfg}"))
+                continue
             s.Solve()
             snap.sync()
             kernel_ms.append(s.last_solve_ms())
@@ -138,11 +164,12 @@ def per_thread(args, mpgos, W, pyoracle):
         cpu_idx = np.arange(0, spec["nt"], 2048)        # stratified: every 2048th instance
         desc = {"system": "duffing_eps", "algorithm": "RKCK45", "rtol": 1e-9, "atol": 1e-9, "events": "x'=0, 256 logged"}
     forms = {}
+    sl = args.slice_steps if args.workload == "duffing_uneven" else 0
     if args.form in ("auto", "plain"):
-        forms["plain"] = time_form(mpgos, spec, {"mode": 0, "block": 64}, args.steps, args.warmup, prec)
+        forms["plain"] = time_form(mpgos, spec, {"mode": 0, "block": 64}, args.steps, args.warmup, prec, sl)
     if args.form in ("auto", "persistent"):
-        forms["persistent"] = time_form(mpgos, spec, {"mode": 1, "block": 256, "round": 32}, args.steps, args.warmup, prec)
-    if args.form == "auto" and not args.no_tune:
+        forms["persistent"] = time_form(mpgos, spec, {"mode": 1, "block": 256, "round": 32}, args.steps, args.warmup, prec, sl)
+    if args.form == "auto" and not args.no_tune and not sl:
         best_cfg = tune(mpgos, spec, prec)
         forms["autotuned"] = time_form(mpgos, spec, best_cfg, args.steps, args.warmup, prec)
     best = min(forms, key=lambda k: forms[k]["wall_ms_per_step"])
@@ -181,7 +208,8 @@ def per_thread(args, mpgos, W, pyoracle):
                       "instances_compared": int(sub["nt"])}
     line = contract_line(args, value, "instance-steps/s", res["wall_ms_per_step"],
                          {"workload": name, "instances": nt, **desc, "kernel_form": best, "launch": res["config"],
-                          "accepted_steps_per_solve": accepted, "rejected_steps_per_solve": attempted - accepted},
+                          "accepted_steps_per_solve": accepted, "rejected_steps_per_solve": attempted - accepted,
+                          "time_sliced_steps": sl or None},
                          {"bound": "mfma", "engine": ENGINE_NOTE, "achieved": achieved,
                           "peak": FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": achieved / (FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS),
@@ -388,6 +416,8 @@ def main():
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "f32k"])
     ap.add_argument("--algorithm", default="rk4", choices=["rk4", "rkck45"], help="duffing_adjoint only")
     ap.add_argument("--checkpoint-interval", type=int, default=64)
+    ap.add_argument("--slice-steps", type=int, default=2000000,
+                    help="duffing_uneven: attempted steps per instance per launch of the time-sliced solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")

@@ -185,6 +185,15 @@ int mpgos_pt_sync_d2h(mpgos_solver* h, int variable);
 /* ProblemSolver::Solve() (SingleSystem_PerThread_Interface.cuh:1279-1285):
  *   asynchronous launch of the integration kernel on the handle's stream. */
 int mpgos_pt_solve(mpgos_solver* h);
+/* New (no reference counterpart): one slice of a Solve.  Every unfinished
+ * instance attempts at most `max_attempted_steps` steps in this launch; an
+ * instance that has not finished is suspended (its whole lane state is kept in
+ * device memory) and resumed by the next call.  The first call of a sequence
+ * starts the Solve, the call that returns *unfinished == 0 completes it, and
+ * the results are bit-identical to mpgos_pt_solve.  For workloads whose
+ * longest instance runs for minutes: the caller regains control between
+ * slices (progress, cancellation).  Synchronous; not with GraphCapture. */
+int mpgos_pt_solve_slice(mpgos_solver* h, long long max_attempted_steps, int* unfinished);
 
 /* ProblemSolver::SynchroniseDevice / InsertSynchronisationPoint / SynchroniseSolver
  *   (SingleSystem_PerThread_Interface.cuh:1288-1312). */

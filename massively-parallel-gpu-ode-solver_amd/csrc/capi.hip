@@ -145,6 +145,12 @@ void* mpgos_pt_device_buffer(mpgos_solver* h, int variable, size_t* nbytes)
 int mpgos_pt_sync_h2d(mpgos_solver* h, int variable) { MPGOS_CHECK_HANDLE(h); return h->pt->sync(variable, true); }
 int mpgos_pt_sync_d2h(mpgos_solver* h, int variable) { MPGOS_CHECK_HANDLE(h); return h->pt->sync(variable, false); }
 int mpgos_pt_solve(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->solve(); }
+int mpgos_pt_solve_slice(mpgos_solver* h, long long max_attempted_steps, int* unfinished)
+{
+    MPGOS_CHECK_HANDLE(h);
+    if (!unfinished) { set_error("mpgos_pt_solve_slice: null pointer"); return MPGOS_E_ARG; }
+    return h->pt->solve_slice(max_attempted_steps, unfinished);
+}
 int mpgos_pt_synchronise_device(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->synchronise_device(); }
 int mpgos_pt_insert_synchronisation_point(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->insert_sync_point(); }
 int mpgos_pt_synchronise_solver(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->synchronise_solver(); }

@@ -54,7 +54,14 @@ struct Args {
     int* DOidx; P* DOt; P* DOx;
     // per-instance outputs
     long long* nacc; long long* nrej; int* status; int* nev; P* evlog; int* evlogidx;
-    unsigned int* queue;      // persistent form: [0] = next instance
+    unsigned int* queue;      // persistent form: [0] = next list entry; [1] = suspended count of a slice
+    // time-sliced solve: the instances to work on are list[0, listLen) (the
+    // identity 0..nActive-1 when list is null); with slice > 0 an instance
+    // that has attempted `slice` steps in this launch is suspended: its lane
+    // state goes to saved[w * NT + id] (32-bit words), phase[id] = 1, and its id
+    // is appended to listOut (count in queue[1]); phase[id] = 2 once finished
+    const int* list; int* listOut; uint32_t* saved; int* phase;
+    int listLen, slice;
     int NT, nActive, NDO, evcap, preferShared;
     int doFreq;
     int roundSteps;           // persistent form: attempted steps between compactions
@@ -85,6 +92,48 @@ struct Lane {
     int doIdx, skipped, term, udt;
     int nacc, nrej, status, nev;
     int tid;
+    int left;                             // time-sliced solve: attempted steps left in this launch
+
+    // whole lane state to and from HBM (time-sliced solve), bit for bit
+    __device__ __forceinline__ void save(const A& a) const
+    {
+        constexpr int NW = (int)(sizeof(Lane) / 4);
+        uint32_t w[NW];
+        __builtin_memcpy(w, this, sizeof(Lane));
+#pragma unroll
+        for (int i = 0; i < NW; i++) a.saved[(size_t)i * a.NT + tid] = w[i];
+    }
+    __device__ __forceinline__ void restore(const A& a, int id)
+    {
+        constexpr int NW = (int)(sizeof(Lane) / 4);
+        uint32_t w[NW];
+#pragma unroll
+        for (int i = 0; i < NW; i++) w[i] = a.saved[(size_t)i * a.NT + id];
+        __builtin_memcpy(this, w, sizeof(Lane));
+    }
+    // the instance's first launch initialises it, later launches resume it
+    __device__ __forceinline__ void begin(const A& a, int id, P* sPAR, int* sPARi)
+    {
+        if (a.phase[id] == 1) restore(a, id);
+        else load(a, id, sPAR, sPARi);
+        left = a.slice;
+    }
+    // after a step: true when the lane is free again (finished or suspended)
+    __device__ __forceinline__ bool settle(const A& a, P* sPAR, int* sPARi)
+    {
+        if (term) {
+            finish(a, sPAR, sPARi);
+            a.phase[tid] = 2;
+            return true;
+        }
+        if (--left <= 0) {
+            save(a);
+            a.phase[tid] = 1;
+            a.listOut[atomicAdd(&a.queue[1], 1u)] = tid;
+            return true;
+        }
+        return false;
+    }
 
     // ------------------------------------------------------------ load / prologue
     __device__ __forceinline__ void load(const A& a, int id, P* sPAR, int* sPARi)
@@ -433,17 +482,26 @@ __device__ __forceinline__ void stage_shared(const Args<Sys, P, CP>& a, P*& sPAR
 }
 
 // Reference form: lane = instance, whole integration in one launch.
-template <class Sys, class P, bool KAHAN, int ALG>
+template <class Sys, class P, bool KAHAN, int ALG, bool SLICED = false>
 __device__ __forceinline__ void plain_body(const Args<Sys, P, typename Lane<Sys, P, KAHAN, ALG>::CP>& a)
 {
     P* sPAR; int* sPARi;
     stage_shared(a, sPAR, sPARi);
-    const int tid = threadIdx.x + blockIdx.x * blockDim.x;
-    if (tid < a.nActive) {
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i < a.listLen) {
         Lane<Sys, P, KAHAN, ALG> L;
-        L.load(a, tid, sPAR, sPARi);
-        while (L.term == 0) L.step(a, sPAR, sPARi);
-        L.finish(a, sPAR, sPARi);
+        const int id = a.list ? a.list[i] : i;
+        if constexpr (SLICED) {
+            L.begin(a, id, sPAR, sPARi);
+            for (;;) {
+                L.step(a, sPAR, sPARi);
+                if (L.settle(a, sPAR, sPARi)) break;
+            }
+        } else {
+            L.load(a, id, sPAR, sPARi);
+            while (L.term == 0) L.step(a, sPAR, sPARi);
+            L.finish(a, sPAR, sPARi);
+        }
     }
 }
 
@@ -458,7 +516,7 @@ __device__ __forceinline__ void plain_body(const Args<Sys, P, typename Lane<Sys,
 // Every instance's arithmetic is independent of the lane that runs it, so the
 // results are bit-identical to the plain form.  The workgroup leaves the loop
 // together once the queue is drained and no lane is live (block-uniform test).
-template <class Sys, class P, bool KAHAN, int ALG>
+template <class Sys, class P, bool KAHAN, int ALG, bool SLICED = false>
 __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane<Sys, P, KAHAN, ALG>::CP>& a)
 {
     using L_t = Lane<Sys, P, KAHAN, ALG>;
@@ -500,8 +558,8 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
             const unsigned need = (unsigned)B - live;
             if (!ctl[18] && need > 0) {
                 base = atomicAdd(a.queue, need);
-                if (base < (unsigned)a.nActive) fresh = min(need, (unsigned)a.nActive - base);
-                if (base + need >= (unsigned)a.nActive) ctl[18] = 1u;
+                if (base < (unsigned)a.listLen) fresh = min(need, (unsigned)a.listLen - base);
+                if (base + need >= (unsigned)a.listLen) ctl[18] = 1u;
             }
             ctl[16] = base;
             ctl[17] = fresh;
@@ -529,7 +587,10 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
             __builtin_memcpy(&L, w, sizeof(L_t));
             busy = true;
         } else if ((unsigned)tid < live + fresh) {
-            L.load(a, (int)(base + ((unsigned)tid - live)), sPAR, sPARi);
+            const int q = (int)(base + ((unsigned)tid - live));
+            const int id = a.list ? a.list[q] : q;
+            if constexpr (SLICED) L.begin(a, id, sPAR, sPARi);
+            else L.load(a, id, sPAR, sPARi);
             busy = true;
         } else {
             busy = false;
@@ -539,7 +600,9 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
             if (!__any(busy)) break;
             if (busy) {
                 L.step(a, sPAR, sPARi);
-                if (L.term) {
+                if constexpr (SLICED) {
+                    if (L.settle(a, sPAR, sPARi)) busy = false;
+                } else if (L.term) {
                     L.finish(a, sPAR, sPARi);
                     busy = false;
                 }

@@ -56,8 +56,11 @@ template <class Sys, class P, bool KAHAN, int ALG, int MODE, int WPE>
 __global__ void __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE)))
 solve_kernel(Args<Sys, P, typename Lane<Sys, P, KAHAN, ALG>::CP> a)
 {
+    // MODE + 2: the same forms with the time-sliced lane lifecycle
     if (MODE == MPGOS_KERNEL_PLAIN) plain_body<Sys, P, KAHAN, ALG>(a);
-    else persistent_body<Sys, P, KAHAN, ALG>(a);
+    else if (MODE == MPGOS_KERNEL_PERSISTENT) persistent_body<Sys, P, KAHAN, ALG>(a);
+    else if (MODE == MPGOS_KERNEL_PLAIN + 2) plain_body<Sys, P, KAHAN, ALG, true>(a);
+    else persistent_body<Sys, P, KAHAN, ALG, true>(a);
 }
 
 // Virtual interface the C ABI talks to; one implementation per (system, precision).
@@ -79,6 +82,7 @@ public:
     virtual int print(int var, int tid, const char* dir) = 0;
     virtual int last_solve_ms(double* ms) const = 0;
     virtual void* stream() = 0;
+    virtual int solve_slice(long long max_attempted_steps, int* unfinished) = 0;
     virtual int adjoint_record(int K, int capacity) = 0;
     virtual int adjoint_reverse() = 0;
 };
@@ -164,6 +168,8 @@ public:
         if (ckpt) (void)hipFree(ckpt);
         if (seg) (void)hipFree(seg);
         if (queue) (void)hipFree(queue);
+        for (void* q : {(void*)saved, (void*)phase, (void*)listA, (void*)listB})
+            if (q) (void)hipFree(q);
         if (syncEvent) (void)hipEventDestroy(syncEvent);
         if (evStart) (void)hipEventDestroy(evStart);
         if (evStop) (void)hipEventDestroy(evStop);
@@ -468,6 +474,7 @@ public:
         a.nacc = nacc.d; a.nrej = nrej.d; a.status = status.d; a.nev = nev.d;
         a.evlog = evlog.d; a.evlogidx = evlogidx.d; a.queue = queue;
         a.NT = NT; a.nActive = nActive; a.NDO = NDO; a.evcap = evcap; a.preferShared = preferShared;
+        a.list = nullptr; a.listLen = nActive; a.slice = 0;
         a.doFreq = doFreq;
         a.roundSteps = roundSteps;
         a.dt0 = (CP)(P)dt0;   // options are stored as Precision in the reference (PT-IF:619-650)
@@ -495,13 +502,19 @@ public:
         }
     }
 
-    KernelFn kernel() const
+    KernelFn kernel(bool sliced = false) const
     {
-        if (algorithm == MPGOS_RK4)
-            return kernelMode == MPGOS_KERNEL_PLAIN ? pick_variant<MPGOS_RK4, MPGOS_KERNEL_PLAIN>(variant)
-                                                    : pick_variant<MPGOS_RK4, MPGOS_KERNEL_PERSISTENT>(variant);
-        return kernelMode == MPGOS_KERNEL_PLAIN ? pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PLAIN>(variant)
-                                                : pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PERSISTENT>(variant);
+        const bool plain = kernelMode == MPGOS_KERNEL_PLAIN;
+        if (algorithm == MPGOS_RK4) {
+            if (sliced) return plain ? pick_variant<MPGOS_RK4, MPGOS_KERNEL_PLAIN + 2>(variant)
+                                     : pick_variant<MPGOS_RK4, MPGOS_KERNEL_PERSISTENT + 2>(variant);
+            return plain ? pick_variant<MPGOS_RK4, MPGOS_KERNEL_PLAIN>(variant)
+                         : pick_variant<MPGOS_RK4, MPGOS_KERNEL_PERSISTENT>(variant);
+        }
+        if (sliced) return plain ? pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PLAIN + 2>(variant)
+                                 : pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PERSISTENT + 2>(variant);
+        return plain ? pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PLAIN>(variant)
+                     : pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PERSISTENT>(variant);
     }
 
     size_t dyn_lds() const
@@ -582,6 +595,67 @@ public:
         return MPGOS_OK;
     }
     int graphGrid = 0, graphBlock = 0;
+
+    // ------------------------------------------------------------ time-sliced solve
+    uint32_t* saved = nullptr;       // lane states of suspended instances [words][NT]
+    int* phase = nullptr;            // per instance: 0 not started, 1 suspended, 2 finished
+    int* listA = nullptr;            // instance lists of consecutive slices
+    int* listB = nullptr;
+    const int* sliceList = nullptr;
+    int sliceLen = 0;
+    bool sliceActive = false;
+
+    static constexpr size_t lane_words()
+    {
+        const size_t a = sizeof(Lane<Sys, P, KAHAN, MPGOS_RK4>), b = sizeof(Lane<Sys, P, KAHAN, MPGOS_RKCK45>);
+        return (a > b ? a : b) / 4;
+    }
+
+    // One slice of a Solve: every unfinished instance attempts at most
+    // max_attempted_steps steps in this launch, then is suspended (lane state
+    // kept in HBM) or finished.  The first call of a sequence starts the Solve
+    // (initialisation), the call that returns *unfinished == 0 completes it;
+    // the results are bit-identical to Solve().  Synchronous.
+    int solve_slice(long long budget, int* unfinished) override
+    {
+        if (budget < 1 || budget > 0x7fffffffLL) { set_error("solve_slice: step budget must be in 1..2^31-1"); return MPGOS_E_ARG; }
+        if (graphCapture) { set_error("solve_slice: not available with GraphCapture"); return MPGOS_E_STATE; }
+        MPGOS_HIP_TRY(hipSetDevice(device));
+        if (!saved) {
+            if (hipMalloc((void**)&saved, lane_words() * 4 * (size_t)NT) != hipSuccess || hipMalloc((void**)&phase, 4 * (size_t)NT) != hipSuccess ||
+                hipMalloc((void**)&listA, 4 * (size_t)NT) != hipSuccess || hipMalloc((void**)&listB, 4 * (size_t)NT) != hipSuccess) {
+                set_error("solve_slice: allocation failed");
+                return MPGOS_E_MEMORY;
+            }
+        }
+        if (!sliceActive) {
+            MPGOS_HIP_TRY(hipMemsetAsync(phase, 0, 4 * (size_t)NT, strm));
+            sliceList = nullptr;
+            sliceLen = nActive;
+            sliceActive = true;
+        }
+        if (sliceLen == 0) { sliceActive = false; *unfinished = 0; return MPGOS_OK; }
+        Args<Sys, P, CP> a = make_args();
+        a.saved = saved; a.phase = phase; a.list = sliceList; a.listLen = sliceLen; a.slice = (int)budget;
+        a.listOut = sliceList == listA ? listB : listA;
+        KernelFn fn = kernel(true);
+        int grid = 0;
+        if (int rc = launch_geometry(fn, grid)) return rc;
+        if (kernelMode == MPGOS_KERNEL_PLAIN) grid = (sliceLen + block() - 1) / block();
+        MPGOS_HIP_TRY(hipMemsetAsync(queue, 0, 8, strm));
+        MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(block()), dyn_lds(), strm, a);
+        MPGOS_HIP_TRY(hipGetLastError());
+        MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
+        unsigned left = 0;
+        MPGOS_HIP_TRY(hipMemcpyAsync(&left, queue + 1, 4, hipMemcpyDeviceToHost, strm));
+        MPGOS_HIP_TRY(hipStreamSynchronize(strm));
+        sliceList = a.listOut;
+        sliceLen = (int)left;
+        if (sliceLen == 0) sliceActive = false;
+        *unfinished = sliceLen;
+        return MPGOS_OK;
+    }
 
     // ------------------------------------------------------------ adjoint (pt_adjoint.h)
     static constexpr bool kAdjoint = Sys::HAS_VJP && std::is_same<P, double>::value && !KAHAN;

@@ -167,6 +167,14 @@ class ProblemSolver:
     def Solve(self):
         check(lib().mpgos_pt_solve(self._h))
 
+    def solve_slice(self, max_attempted_steps):
+        """One slice of a Solve (synchronous): every unfinished instance attempts
+        at most `max_attempted_steps` steps; returns the number of instances
+        still unfinished (0 completes the Solve, bit-identical to Solve())."""
+        left = ctypes.c_int()
+        check(lib().mpgos_pt_solve_slice(self._h, int(max_attempted_steps), ctypes.byref(left)))
+        return left.value
+
     def SynchroniseDevice(self):
         check(lib().mpgos_pt_synchronise_device(self._h))
 

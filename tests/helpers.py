@@ -6,7 +6,7 @@ from mpgos import workloads as W
 
 
 def run_gpu(spec, precision=Precision.DOUBLE, kernel_mode=0, block=None, variant=0, graph=0, solves=1,
-            waves_per_simd=0):
+            waves_per_simd=0, slice_steps=0):
     s = ProblemSolver(spec["system"], spec["nt"], spec["algorithm"], precision, ndo=spec.get("ndo", 0))
     try:
         W.load(s, spec)
@@ -18,8 +18,19 @@ def run_gpu(spec, precision=Precision.DOUBLE, kernel_mode=0, block=None, variant
             s.SolverOption(O.ThreadsPerBlock, block)
         outs = []
         for _ in range(solves):
-            s.solve_blocking()
-            outs.append(W.read(s))
+            if slice_steps:
+                slices = 1
+                while s.solve_slice(slice_steps) > 0:
+                    slices += 1
+                s.SynchroniseFromDeviceToHost(V.All)
+                s.InsertSynchronisationPoint()
+                s.SynchroniseSolver()
+                out = W.read(s)
+                out["_slices"] = slices
+                outs.append(out)
+            else:
+                s.solve_blocking()
+                outs.append(W.read(s))
         return outs if solves > 1 else outs[0]
     finally:
         s.close()

@@ -45,6 +45,31 @@ def test_config2_events_subset(gpu, oracle):
     assert ref["nev"].sum() > 0
 
 
+@pytest.mark.parametrize("mode", [0, 1], ids=["plain", "persistent"])
+def test_time_sliced_solve_is_bitwise(gpu, oracle, mode):
+    """Solve in slices of a few attempted steps (lane states suspended to HBM
+    and resumed): bit-identical to the oracle, events and logs included."""
+    full = W.config2_duffing_uneven(nt=131072, evcap=256)
+    idx = np.argsort(full["TD"][1])[:700]
+    spec = W.subset(full, idx)
+    spec["n_active"] = 650
+    print(f"sliced solve mode={mode}", flush=True)
+    ref = oracle.run("duffing_eps", spec)
+    out = run_gpu(spec, kernel_mode=mode, slice_steps=997)
+    assert out["_slices"] > 3
+    assert_bitwise(out, ref, keys=("X", "T", "nacc", "nrej", "status", "nev", "evlog", "evlogidx"), label="sliced")
+
+
+@pytest.mark.parametrize("name", ["dense_output_f3_m0.0", "tutorial_duffing_doublewell", "event_ramp_both"])
+def test_time_sliced_scenarios(gpu, oracle, name):
+    _, system, spec = next(x for x in S.all_scenarios() if x[0] == name)
+    ref = oracle.run(system, spec)
+    for mode in (0, 1):
+        out = run_gpu(spec, kernel_mode=mode, slice_steps=3)
+        keys = ("X", "T", "TD", "cPAR", "ACC", "ACCi", "nacc", "nrej", "status", "nev", "DOidx", "DOt", "DOx")
+        assert_bitwise(out, ref, keys=keys, label=f"{name} sliced mode {mode}")
+
+
 # ---------------------------------------------------------------- scenario sweep
 import scenarios as S  # noqa: E402
 
