@@ -161,7 +161,7 @@ def per_thread(args, mpgos, W, pyoracle):
     else:
         spec = W.config2_duffing_uneven(nt=args.nt or 131072)
         name = f"duffing_uneven_rkck45_{spec['nt']}"
-        cpu_idx = np.arange(0, spec["nt"], 2048)        # stratified: every 2048th instance
+        cpu_idx = np.arange(0, spec["nt"], 512)         # stratified: every 512th instance
         desc = {"system": "duffing_eps", "algorithm": "RKCK45", "rtol": 1e-9, "atol": 1e-9, "events": "x'=0, 256 logged"}
     forms = {}
     sl = args.slice_steps if args.workload == "duffing_uneven" else 0

@@ -626,7 +626,7 @@ __device__ void pb_body(const Args<Sys, P>& a)
                     if (n < a.recStepCap) { a.recT[(size_t)n * NS + gsys] = my->t; a.recH[(size_t)n * NS + gsys] = my->dt; }
                     else my->status |= MPGOS_STATUS_CHECKPOINT_OVERFLOW;
                 }
-                if ((n + 1) % a.recK == 0) {
+                if ((n + 1) % a.recK == 0) {       // once per accepted step of the block, not per stage
                     const long long k = (n + 1) / a.recK;
                     if (k < a.recCkCap) {
 #pragma unroll

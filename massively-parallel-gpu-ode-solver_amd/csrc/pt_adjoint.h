@@ -74,13 +74,17 @@ __device__ __forceinline__ void record_body(const AdjArgs<Sys>& A)
     Lane<Sys, double, false, ALG> L;
     L.load(a, tid, sPAR, sPARi);
     store_checkpoint<Sys, ALG>(A, L, 0);
+    int untilCk = A.K, k = 0;      // accepted steps to the next checkpoint (no division in the loop)
     while (L.term == 0) {
         const int before = L.nacc;
         L.step(a, sPAR, sPARi);
-        if (L.nacc != before && L.term == 0 && (L.nacc % A.K) == 0) {
-            const int k = L.nacc / A.K;
-            if (k < A.ckcap) store_checkpoint<Sys, ALG>(A, L, k);
-            else L.status |= MPGOS_STATUS_CHECKPOINT_OVERFLOW_BIT;
+        if (L.nacc != before && --untilCk == 0) {
+            untilCk = A.K;
+            ++k;
+            if (L.term == 0) {
+                if (k < A.ckcap) store_checkpoint<Sys, ALG>(A, L, k);
+                else L.status |= MPGOS_STATUS_CHECKPOINT_OVERFLOW_BIT;
+            }
         }
     }
     L.finish(a, sPAR, sPARi);

@@ -36,7 +36,10 @@ static void run_all(const Job& j)
     for (int i = 0; i < Sys::NSP; i++) sp[i] = S(j.sPAR[i]);
     for (int i = 0; i < Sys::NISP; i++) spi[i] = j.sPARi[i];
     const int threads = j.nthreads > 0 ? j.nthreads : omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+    // chunks of up to 64 instances, small enough that every thread gets work
+    int chunk = j.n_active / (threads * 8);
+    chunk = chunk < 1 ? 1 : (chunk > 64 ? 64 : chunk);
+#pragma omp parallel for schedule(dynamic, chunk) num_threads(threads)
     for (int tid = 0; tid < j.n_active; tid++) {
         std::vector<S> spl(sp);   // per-thread copy: callbacks may not write it, but keep lanes independent
         std::vector<int> spil(spi);
