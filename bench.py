@@ -350,7 +350,57 @@ def ring(args, mpgos, W, pyoracle):
                          cpu, parity)
     line["forms"] = {k: {"kernel_ms": v["kernel_ms"], "system_steps_per_s": accepted / (v["kernel_ms"] * 1e-3)}
                      for k, v in forms.items()}
+    if not args.no_adjoint:
+        line["adjoint"] = ring_adjoint(args, W, ns, U, kernels[best])
     return line
+
+
+def ring_adjoint(args, W, ns, U, kernel):
+    """Cost of the per-block discrete adjoint of the ring: the same rings with a
+    quadrature unit state q' = x^2 (the loss is a combination of q at the
+    ends of the last forcing period), gradient w.r.t. the per-system forcing
+    amplitude B.  Forward Solve, record pass and reverse pass of the same
+    system, each timed with HIP events on the solver's stream."""
+    from mpgos.perblock import PB, PBOption, PBProblemSolver
+    spec = W.config4_ring(ns=ns, ups=U, periods=4, system="ring_duffing_q")
+    K = args.checkpoint_interval
+    s = PBProblemSolver("ring_duffing_q", ns, U, spec["algorithm"])
+    fwd, rec, rev = [], [], []
+    try:
+        s.SolverOption(PBOption.CouplingKernel, kernel)
+        W.load_pb(s, spec)
+        s.solve_blocking()
+        nmax = int(W.read_pb(s)["nacc"].max())
+        lam = np.zeros((3, ns * U))
+        lam[2] = 1.0 / (2.0 * np.pi * U)                   # d(mean over units of q(end))/dq / period
+        for i in range(args.warmup + args.steps):
+            W.load_pb(s, spec)
+            s.Solve()
+            s.SynchroniseDevice()
+            f_ms = s.last_solve_ms()
+            W.load_pb(s, spec)
+            s.adjoint_record(K, nmax // K + 2, nmax + 2)
+            s.SynchroniseDevice()
+            r_ms = s.last_solve_ms()
+            s.host_array(PB.AdjointState)[...] = lam
+            s.SynchroniseFromHostToDevice(PB.AdjointState)
+            s.adjoint_reverse()
+            s.SynchroniseDevice()
+            b_ms = s.last_solve_ms()
+            log(f"ring adjoint rep {i}: forward {f_ms:.1f} ms, record {r_ms:.1f} ms, reverse {b_ms:.1f} ms")
+            if i >= args.warmup:
+                fwd.append(f_ms); rec.append(r_ms); rev.append(b_ms)
+        s.SynchroniseFromDeviceToHost(PB.SystemParameterGradient)
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
+        gB = s.host_array(PB.SystemParameterGradient)[0].copy()
+    finally:
+        s.close()
+    f, r, b = float(np.mean(fwd)), float(np.mean(rec)), float(np.mean(rev))
+    return {"system": "ring_duffing_q (unit state x, v, q with q' = x^2)", "gradient": "dL/dB per system",
+            "checkpoint_interval": K, "forward_ms": f, "record_ms": r, "reverse_ms": b,
+            "adjoint_over_forward": (r + b) / f, "reverse_over_forward": b / f,
+            "gradient_finite": bool(np.isfinite(gB).all())}
 
 
 # --------------------------------------------------------------------------- common
@@ -420,6 +470,7 @@ def main():
                     help="duffing_uneven: attempted steps per instance per launch of the time-sliced solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--no-adjoint", action="store_true", help="ring: skip the per-block adjoint timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -39,7 +39,8 @@ namespace pb {
 
 template <class Sys>
 struct AdjArgs {
-    double* seg;          // [K][UD][NU]
+    double* seg;          // [K][UD][NU] (lane form)
+    double* stg;          // [K][stages][UD + NC][NU] stage inputs and coupling values (dense-contraction form)
     double* lam;          // in: dL/dx(end), out: dL/dx(start)      [UD][NU]
     double* gU;           // out: dL/d(unit parameters)             [NUP][NU]
     double* gS;           // out: dL/d(system parameters)           [NSP][NS]
@@ -410,6 +411,430 @@ template <class Sys, int ALG, int MAXT>
 __global__ void __launch_bounds__(MAXT) pb_reverse_kernel(Args<Sys, double> a, AdjArgs<Sys> A)
 {
     pb_reverse_body<Sys, ALG>(a, A);
+}
+
+// ---------------------------------------------------------------- dense-contraction form
+// Butcher coefficients of the steppers (the adjoint's coefficients; the forward
+// arithmetic itself is written out in the steps below) and the stage times
+// exactly as the forward pass computes them.
+template <int ALG>
+__device__ __forceinline__ double rk_b(int s)
+{
+    if (ALG == MPGOS_RK4) return (s == 0 || s == 3) ? 1.0 / 6.0 : 2.0 / 6.0;
+    switch (s) {
+    case 0: return 37.0 / 378.0;
+    case 2: return 250.0 / 621.0;
+    case 3: return 125.0 / 594.0;
+    case 5: return 512.0 / 1771.0;
+    default: return 0.0;
+    }
+}
+
+template <int ALG>
+__device__ __forceinline__ double rk_a(int s, int l)
+{
+    if (ALG == MPGOS_RK4) return (l == s - 1) ? (s == 3 ? 1.0 : 0.5) : 0.0;
+    constexpr double A[6][5] = {{0, 0, 0, 0, 0},
+                                {1.0 / 5.0, 0, 0, 0, 0},
+                                {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0},
+                                {3.0 / 10.0, -9.0 / 10.0, 6.0 / 5.0, 0, 0},
+                                {-11.0 / 54.0, 5.0 / 2.0, -70.0 / 27.0, 35.0 / 27.0, 0},
+                                {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}};
+    return l < s ? A[s][l] : 0.0;
+}
+
+template <int ALG>
+__device__ __forceinline__ double rk_stage_time(int s, double t, double h)
+{
+    using math::fma;
+    if (ALG == MPGOS_RK4) return s == 0 ? t : (s == 3 ? t + h : t + 0.5 * h);
+    switch (s) {
+    case 0: return t;
+    case 1: return fma(1.0 / 5.0, h, t);
+    case 2: return fma(3.0 / 10.0, h, t);
+    case 3: return fma(3.0 / 5.0, h, t);
+    case 4: return t + h;
+    default: return fma(7.0 / 8.0, h, t);
+    }
+}
+
+// LDS of the dense-contraction reverse pass: system scopes, the B operand
+// [coupling][unit][16] (columns 8..15 stay zero), the contraction results
+// [coupling][unit][system], the system-parameter gradient partials and two
+// block-wide counters.
+struct RevMfmaLayout { size_t cptB, coup, red, blk, total; };
+
+template <class Sys>
+__host__ __device__ inline RevMfmaLayout reverse_mfma_layout(int UPS)
+{
+    constexpr int NCp = Pos<Sys::NC>::value, NSPp = Pos<Sys::NSP>::value, SPB = kMfmaSystems;
+    RevMfmaLayout L;
+    size_t o = ((size_t)SPB * sizeof(RevScope<Sys>) + 15) & ~(size_t)15;
+    L.cptB = o;
+    o += (size_t)NCp * UPS * 16 * sizeof(double);
+    L.coup = o;
+    o += (size_t)NCp * UPS * SPB * sizeof(double);
+    L.red = o;
+    o += (size_t)SPB * UPS * NSPp * sizeof(double);
+    L.blk = o;
+    o += 2 * sizeof(int);
+    L.total = o;
+    return L;
+}
+
+// Reverse pass in the dense-contraction form (full FP64 coupling matrix, UPS a
+// multiple of 32 up to 256): 8 systems per workgroup, 4 units per lane
+// (2 * UPS threads), every coupling product on v_mfma_f64_16x16x4f64 --
+// M CPT(Y) in the segment recompute, M^T w (cMatT) in the adjoint sweep.
+// The segment recompute keeps each stage's input Y_s and coupling values v_s
+// of every step in the stage buffer
+//   stg[(((j * ST + s) * (UD + NC)) + c) * NU + unit],  j < K,
+// so the adjoint sweep evaluates no forward coupling product again; the
+// coupling factors CPF_s come from the unit's right-hand side at (Y_s, T_s).
+// Sums run in the same order as in the lane form (the accumulation chain of
+// the MFMA instruction is a sequential fused multiply-add in column order),
+// so both forms return the same bits.
+template <class Sys, int ALG, int PAIRS>
+__device__ void pb_reverse_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys>& A)
+{
+    using P = double;
+    constexpr int UD = Sys::UD, NC = Sys::NC, NUP = Sys::NUP, NSP = Sys::NSP;
+    constexpr int NUA = Sys::NUA, NiUA = Sys::NiUA, NSA = Sys::NSA, NiSA = Sys::NiSA;
+    constexpr int NCp = Pos<NC>::value, NUPp = Pos<NUP>::value, NSPp = Pos<NSP>::value;
+    constexpr int ST = ALG == MPGOS_RK4 ? 4 : 6;
+    constexpr int SPB = kMfmaSystems;
+    constexpr int SW = UD + NC;                        // stage-buffer words per unit and stage
+    using math::fma;
+    using Scope = RevScope<Sys>;
+    using U = Unit<Sys, P>;
+
+    extern __shared__ __align__(16) unsigned char mpgos_pb_lds[];
+    const int UPS = a.UPS, NS = a.NS, K = a.recK;
+    const RevMfmaLayout L = reverse_mfma_layout<Sys>(UPS);
+    Scope* sys = reinterpret_cast<Scope*>(mpgos_pb_lds);
+    P* cptB = reinterpret_cast<P*>(mpgos_pb_lds + L.cptB);
+    P* coup = reinterpret_cast<P*>(mpgos_pb_lds + L.coup);
+    P* red = reinterpret_cast<P*>(mpgos_pb_lds + L.red);             // [SPB][UPS][NSPp]
+    int* blk = reinterpret_cast<int*>(mpgos_pb_lds + L.blk);          // [0] rounds, [1] longest segment
+
+    const int tid = threadIdx.x;
+    const int lanesPerSys = UPS / PAIRS;
+    const int lsys = tid / lanesPerSys;                                 // < SPB (blockDim.x = SPB * UPS / PAIRS)
+    const int u0 = tid - lsys * lanesPerSys;
+    const bool leader = u0 == 0;
+    const int gsys = blockIdx.x * SPB + lsys;
+    const bool active = gsys < NS && gsys < a.nActive;
+    const size_t NU = (size_t)NS * UPS;
+    Scope* my = &sys[lsys];
+    auto uid = [&](int p) { return u0 + p * lanesPerSys; };
+    auto gu = [&](int p) { return (size_t)gsys * UPS + uid(p); };
+    // unit-scope rows [row][NU]: the row offset is wave-uniform (scalar base
+    // address), the unit a 32-bit per-lane offset
+    const unsigned gu0 = (unsigned)gsys * (unsigned)UPS + (unsigned)u0;
+    auto row = [&](double* base, size_t r) { return base + r * NU; };
+    auto at = [&](int p) { return gu0 + (unsigned)(p * lanesPerSys); };
+
+    int cIndex[NCp];
+#pragma unroll
+    for (int i = 0; i < NC; i++) cIndex[i] = a.cIndex[i];
+    for (int i = tid; i < NCp * UPS * 16; i += blockDim.x) cptB[i] = P(0);
+    if (tid < 2) blk[tid] = 0;
+    if (leader) {
+        my->status = 0;
+        my->nseg = 0;
+        if (active) {
+            for (int i = 0; i < NSP; i++) my->sPAR[i] = a.sPAR[gsys + i * NS];
+            for (int i = 0; i < NSA; i++) my->sACC[i] = a.sACC[gsys + i * NS];
+            for (int i = 0; i < NiSA; i++) my->sACCi[i] = a.sACCi[gsys + i * NS];
+            for (int i = 0; i < NC; i++) my->strength[i] = a.strength[gsys + i * NS];
+            my->nacc = a.nacc[gsys];
+            my->status = a.status[gsys];
+            const bool ok = (my->status & MPGOS_STATUS_CHECKPOINT_OVERFLOW) == 0;
+            my->nseg = ok ? (int)((my->nacc + K - 1) / K) : 0;
+        }
+    }
+    P uPAR[PAIRS][NUPp], uACC[PAIRS][Pos<NUA>::value], lam[PAIRS][UD], muU[PAIRS][NUPp], muS[PAIRS][NSPp];
+    int uACCi[PAIRS][Pos<NiUA>::value];
+#pragma unroll
+    for (int p = 0; p < PAIRS; p++) {
+        const size_t g = gu(p);
+#pragma unroll
+        for (int c = 0; c < NUP; c++) { uPAR[p][c] = active ? a.uPAR[g + c * NU] : P(0); muU[p][c] = P(0); }
+#pragma unroll
+        for (int c = 0; c < NUA; c++) uACC[p][c] = active ? a.uACC[g + c * NU] : P(0);
+#pragma unroll
+        for (int c = 0; c < NiUA; c++) uACCi[p][c] = active ? a.uACCi[g + c * NU] : 0;
+#pragma unroll
+        for (int c = 0; c < UD; c++) lam[p][c] = active ? A.lam[g + c * NU] : P(0);
+#pragma unroll
+        for (int c = 0; c < NSP; c++) muS[p][c] = P(0);
+    }
+    __syncthreads();
+    if (leader) atomicMax(&blk[0], my->nseg);
+    __syncthreads();
+    const int rounds = blk[0];
+    P* gPAR = a.gPAR;
+    int* gPARi = a.gPARi;
+
+    // right-hand sides of the lane's units at (Y, T) plus the coupling terms;
+    // v receives the coupling values (M CPT)_u
+    auto rhs = [&](bool run, P (*F)[UD], P (*Y)[UD], const P T, P (*v)[NCp]) {
+        P CPF[PAIRS][NCp];
+        if (run) {
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++) {
+                P cp[NCp];
+                Sys::template OdeFunction<P>(gsys, uid(p), F[p], Y[p], T, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
+                                             my->sACC, my->sACCi, cp, CPF[p]);
+#pragma unroll
+                for (int i = 0; i < NC; i++) cptB[((size_t)i * UPS + uid(p)) * 16 + lsys] = cp[i];
+            }
+        }
+        __syncthreads();
+        mfma_contract<NC, 4>(a.cMat, cptB, coup, UPS, SPB);
+        __syncthreads();
+        if (run) {
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int i = 0; i < NC; i++) {
+                    v[p][i] = coup[((size_t)i * UPS + uid(p)) * SPB + lsys];
+                    U::add(cIndex[i], my->strength[i] * CPF[p][i], v[p][i], F[p]);
+                }
+        }
+    };
+    // one forward step of segment step j (the forward pass's arithmetic),
+    // keeping Y_s and v_s in the stage buffer
+    auto forward_step = [&](bool run, P (*X)[UD], const P t, const P h, int j) {
+        auto keep = [&](int s, P (*Y)[UD], P (*v)[NCp]) {
+            if (run) {
+#pragma unroll
+                for (int p = 0; p < PAIRS; p++) {
+                    const size_t base = ((size_t)j * ST + s) * SW;
+#pragma unroll
+                    for (int c = 0; c < UD; c++) row(A.stg, base + c)[at(p)] = Y[p][c];
+#pragma unroll
+                    for (int i = 0; i < NC; i++) row(A.stg, base + UD + i)[at(p)] = v[p][i];
+                }
+            }
+        };
+        P v[PAIRS][NCp], next[PAIRS][UD], Y[PAIRS][UD];
+        if (ALG == MPGOS_RK4) {
+            P k[PAIRS][UD];
+            rhs(run, next, X, t, v);
+            keep(0, X, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) Y[p][c] = fma(next[p][c] * P(0.5), h, X[p][c]);
+            const P t2 = t + P(0.5) * h;
+            rhs(run, k, Y, t2, v);
+            keep(1, Y, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) { next[p][c] = fma(P(2), k[p][c], next[p][c]); Y[p][c] = fma(k[p][c] * P(0.5), h, X[p][c]); }
+            rhs(run, k, Y, t2, v);
+            keep(2, Y, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) { next[p][c] = fma(P(2.0), k[p][c], next[p][c]); Y[p][c] = fma(k[p][c], h, X[p][c]); }
+            rhs(run, k, Y, t + h, v);
+            keep(3, Y, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) next[p][c] = fma(h * (next[p][c] + k[p][c]), P(1.0 / 6.0), X[p][c]);
+        } else {
+            P k1[PAIRS][UD], k2[PAIRS][UD], k3[PAIRS][UD], k4[PAIRS][UD], k5[PAIRS][UD], k6[PAIRS][UD];
+            rhs(run, k1, X, t, v);
+            keep(0, X, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) Y[p][c] = fma(k1[p][c] * P(1.0 / 5.0), h, X[p][c]);
+            rhs(run, k2, Y, fma(P(1.0 / 5.0), h, t), v);
+            keep(1, Y, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) Y[p][c] = fma(fma(k2[p][c], P(9.0 / 40.0), k1[p][c] * P(3.0 / 40.0)), h, X[p][c]);
+            rhs(run, k3, Y, fma(P(3.0 / 10.0), h, t), v);
+            keep(2, Y, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++)
+                    Y[p][c] = fma(fma(k3[p][c], P(6.0 / 5.0), fma(k2[p][c], P(-9.0 / 10.0), k1[p][c] * P(3.0 / 10.0))), h, X[p][c]);
+            rhs(run, k4, Y, fma(P(3.0 / 5.0), h, t), v);
+            keep(3, Y, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++)
+                    Y[p][c] = fma(fma(k4[p][c], P(35.0 / 27.0), fma(k3[p][c], P(-70.0 / 27.0), fma(k2[p][c], P(5.0 / 2.0),
+                                  k1[p][c] * P(-11.0 / 54.0)))), h, X[p][c]);
+            rhs(run, k5, Y, t + h, v);
+            keep(4, Y, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++)
+                    Y[p][c] = fma(fma(k5[p][c], P(253.0 / 4096.0), fma(k4[p][c], P(44275.0 / 110592.0), fma(k3[p][c], P(575.0 / 13824.0),
+                                  fma(k2[p][c], P(175.0 / 512.0), k1[p][c] * P(1631.0 / 55296.0))))), h, X[p][c]);
+            rhs(run, k6, Y, fma(P(7.0 / 8.0), h, t), v);
+            keep(5, Y, v);
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++)
+                    next[p][c] = fma(fma(k6[p][c], P(512.0 / 1771.0), fma(k4[p][c], P(125.0 / 594.0), fma(k3[p][c], P(250.0 / 621.0),
+                                     k1[p][c] * P(37.0 / 378.0)))), h, X[p][c]);
+        }
+        if (run) {
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) X[p][c] = next[p][c];
+        }
+    };
+
+    for (int r = 0; r < rounds; r++) {
+        // this round's segment of each system: the r-th from the end
+        if (leader) {
+            const int k = my->nseg - 1 - r;
+            my->k = k;
+            my->len = k >= 0 ? (int)((my->nacc - (long long)k * K) < K ? (my->nacc - (long long)k * K) : K) : 0;
+        }
+        if (tid == 0) blk[1] = 0;
+        __syncthreads();
+        if (leader) atomicMax(&blk[1], my->len);
+        __syncthreads();
+        const int maxlen = blk[1];
+        const int k = my->k, len = my->len;
+        P X[PAIRS][UD];
+#pragma unroll
+        for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+            for (int c = 0; c < UD; c++) X[p][c] = (active && len > 0) ? row(a.recX, (size_t)k * UD + c)[at(p)] : P(0);
+        // 1. recompute the segment, keeping every stage's input and coupling values
+        for (int j = 0; j < maxlen; j++) {
+            const bool run = active && j < len;
+            P t = P(0), h = P(0);
+            if (run) {
+                const size_t n = (size_t)k * K + j;
+                t = a.recT[n * NS + gsys];
+                h = a.recH[n * NS + gsys];
+            }
+            forward_step(run, X, t, h, j);
+        }
+        // 2. adjoint sweep through the segment
+        for (int j = maxlen - 1; j >= 0; j--) {
+            const bool run = active && j < len;
+            P t = P(0), h = P(0);
+            if (run) {
+                const size_t n = (size_t)k * K + j;
+                t = a.recT[n * NS + gsys];
+                h = a.recH[n * NS + gsys];
+            }
+            P kb[ST][PAIRS][UD];
+#pragma unroll
+            for (int s = 0; s < ST; s++)
+#pragma unroll
+                for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                    for (int c = 0; c < UD; c++) kb[s][p][c] = h * rk_b<ALG>(s) * lam[p][c];
+#pragma unroll
+            for (int s = ST - 1; s >= 0; s--) {
+                const P Ts = rk_stage_time<ALG>(s, t, h);
+                const size_t base = ((size_t)j * ST + s) * SW;
+                P kbCPF[PAIRS][NCp];
+                if (run) {
+#pragma unroll
+                    for (int p = 0; p < PAIRS; p++) {
+                        P Y[UD], F[UD], cp[NCp], cpf[NCp];
+#pragma unroll
+                        for (int c = 0; c < UD; c++) Y[c] = row(A.stg, base + c)[at(p)];
+                        Sys::template OdeFunction<P>(gsys, uid(p), F, Y, Ts, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
+                                                     my->sACC, my->sACCi, cp, cpf);
+#pragma unroll
+                        for (int i = 0; i < NC; i++) {
+                            P g = P(0);
+#pragma unroll
+                            for (int c = 0; c < UD; c++)
+                                if (c == cIndex[i]) g = kb[s][p][c];
+                            cptB[((size_t)i * UPS + uid(p)) * 16 + lsys] = my->strength[i] * cpf[i] * g;
+                            kbCPF[p][i] = my->strength[i] * row(A.stg, base + UD + i)[at(p)] * g;
+                        }
+                    }
+                }
+                __syncthreads();
+                mfma_contract<NC, 4>(A.cMatT, cptB, coup, UPS, SPB);
+                __syncthreads();
+#pragma unroll
+                for (int p = 0; p < PAIRS; p++) {
+                    P gy[UD];
+                    if (run) {
+                        P Y[UD], kbCPT[NCp], gU[NUPp], gS[NSPp];
+#pragma unroll
+                        for (int c = 0; c < UD; c++) Y[c] = row(A.stg, base + c)[at(p)];   // reloaded rather than held across the contraction
+#pragma unroll
+                        for (int i = 0; i < NC; i++) kbCPT[i] = coup[((size_t)i * UPS + uid(p)) * SPB + lsys];
+                        Sys::template OdeVJP<P>(gsys, uid(p), kb[s][p], kbCPT, kbCPF[p], Y, Ts, uPAR[p], my->sPAR, gPAR, gPARi,
+                                                uACC[p], uACCi[p], my->sACC, my->sACCi, gy, gU, gS);
+#pragma unroll
+                        for (int c = 0; c < NUP; c++) muU[p][c] += gU[c];
+#pragma unroll
+                        for (int c = 0; c < NSP; c++) muS[p][c] += gS[c];
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < UD; c++) gy[c] = P(0);
+                    }
+#pragma unroll
+                    for (int c = 0; c < UD; c++) lam[p][c] += gy[c];
+#pragma unroll
+                    for (int l = 0; l < ST; l++)
+                        if (l < s && rk_a<ALG>(s, l) != P(0)) {
+#pragma unroll
+                            for (int c = 0; c < UD; c++) kb[l][p][c] = fma(h * rk_a<ALG>(s, l), gy[c], kb[l][p][c]);
+                        }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // outputs: dL/dx(start), dL/d(unit parameters), dL/d(system parameters)
+    const bool bad = active && (my->status & MPGOS_STATUS_CHECKPOINT_OVERFLOW) != 0;
+    if (active) {
+        const P nan = __builtin_nan("");
+#pragma unroll
+        for (int p = 0; p < PAIRS; p++) {
+#pragma unroll
+            for (int c = 0; c < UD; c++) A.lam[gu(p) + c * NU] = bad ? nan : lam[p][c];
+#pragma unroll
+            for (int c = 0; c < NUP; c++) A.gU[gu(p) + c * NU] = bad ? nan : muU[p][c];
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+        for (int c = 0; c < NSP; c++) red[((size_t)lsys * UPS + uid(p)) * NSPp + c] = muS[p][c];
+    __syncthreads();
+    if (leader && active) {
+        for (int c = 0; c < NSP; c++) {
+            P s = P(0);
+            for (int v = 0; v < UPS; v++) s += red[((size_t)lsys * UPS + v) * NSPp + c];
+            A.gS[gsys + (size_t)c * NS] = bad ? __builtin_nan("") : s;
+        }
+    }
+}
+
+template <class Sys, int ALG, int PAIRS>
+__global__ void __launch_bounds__(kMfmaSystems * 256 / PAIRS) pb_reverse_mfma_kernel(Args<Sys, double> a, AdjArgs<Sys> A)
+{
+    pb_reverse_mfma_body<Sys, ALG, PAIRS>(a, A);
 }
 
 } // namespace pb

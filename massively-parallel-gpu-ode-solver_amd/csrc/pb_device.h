@@ -211,10 +211,9 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 // accumulated in column order.  Called by every lane of the workgroup
 // (v_mfma needs all 64 lanes of a wave: blockDim.x is a multiple of 64); UPS is
 // a multiple of 16.
-template <int NC>
+template <int NC, int G = 8>    // G: column blocks per load group (UPS % (4 G) == 0)
 __device__ __forceinline__ void mfma_contract(const double* M, const double* cptB, double* coup, int UPS, int SPB)
 {
-    constexpr int G = 8;                                  // column blocks per load group (UPS % 32 == 0)
     const int l = threadIdx.x & 63;
     const int nw = blockDim.x >> 6;
     const int tiles = UPS >> 4;

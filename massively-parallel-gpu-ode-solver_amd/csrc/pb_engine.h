@@ -82,14 +82,15 @@ public:
     static constexpr bool kAdjoint = std::is_same<P, double>::value && HasVjp<Sys>::value;
     Buffer<P> lamB, gUB, gSB;           // dL/dx, dL/d(unit parameters), dL/d(system parameters)
     double* recX = nullptr; double* recT = nullptr; double* recH = nullptr; double* seg = nullptr; double* cMatT = nullptr;
-    size_t recXn = 0, recTn = 0, recHn = 0, segn = 0, cMatTn = 0;
+    double* stg = nullptr;
+    size_t recXn = 0, recTn = 0, recHn = 0, segn = 0, cMatTn = 0, stgn = 0;
     int adjK = 0, adjCap = 0, adjStepCap = 0;
 
     ~Solver() override
     {
         if (strm) (void)hipStreamSynchronize(strm);
         lamB.release(); gUB.release(); gSB.release();
-        for (double* q : {recX, recT, recH, seg, cMatT})
+        for (double* q : {recX, recT, recH, seg, cMatT, stg})
             if (q) (void)hipFree(q);
         for (auto* b : {&X, &uPAR, &uACC, &TD, &T, &sPAR, &sACC, &strength, &DOt, &DOx, &gPAR, &cMat}) b->release();
         for (auto* b : {&uACCi, &sACCi, &DOidx, &gPARi, &cIndex, &status}) b->release();
@@ -543,8 +544,18 @@ public:
             MPGOS_HIP_TRY(hipSetDevice(device));
             const size_t NU = (size_t)NS * UPS;
             MPGOS_HIP_TRY(hipStreamSynchronize(strm));
+            const int cpl = coupling_form();
+            if (cpl == kMfmaCoupling && !mfma_applies()) {
+                set_error("per-block: the dense-contraction coupling kernel needs a full FP64 coupling matrix and UPS a multiple of 32 up to 256");
+                return MPGOS_E_ARG;
+            }
+            const int ST = algorithm == MPGOS_RK4 ? 4 : 6;
             int rc;
-            if ((rc = grow_device(seg, segn, (size_t)adjK * UD * NU, "segment buffer"))) return rc;
+            if (cpl == kMfmaCoupling) {
+                if ((rc = grow_device(stg, stgn, (size_t)adjK * ST * (UD + NC) * NU, "stage buffer"))) return rc;
+            } else if ((rc = grow_device(seg, segn, (size_t)adjK * UD * NU, "segment buffer"))) {
+                return rc;
+            }
             // transposed coupling matrices in the stored form (from the host mirror)
             std::vector<double> mt((size_t)NC * cMatSize, 0.0);
             for (int i = 0; i < NC; i++)
@@ -558,7 +569,21 @@ public:
             Args<Sys, double> a = make_args();
             a.recX = recX; a.recT = recT; a.recH = recH; a.recK = adjK; a.recCkCap = adjCap; a.recStepCap = adjStepCap;
             AdjArgs<Sys> A;
-            A.seg = seg; A.lam = lamB.d; A.gU = gUB.d; A.gS = gSB.d; A.cMatT = cMatT;
+            A.seg = seg; A.stg = stg; A.lam = lamB.d; A.gU = gUB.d; A.gS = gSB.d; A.cMatT = cMatT;
+            lastForm = cpl;
+            if (cpl == kMfmaCoupling) {
+                a.SPB = kMfmaSystems;
+                const size_t lds = reverse_mfma_layout<Sys>(UPS).total;
+                if (lds > ldsMax) { set_error("per-block adjoint: reverse-pass LDS exceeds a workgroup's"); return MPGOS_E_ARG; }
+                const int threads = kMfmaSystems * UPS / kMfmaPairs;
+                const int grid = NS / kMfmaSystems + (NS % kMfmaSystems == 0 ? 0 : 1);
+                MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_reverse_mfma_kernel<Sys, MPGOS_RK4, kMfmaPairs>), dim3(grid), dim3(threads), lds, strm, a, A);
+                else hipLaunchKernelGGL((pb_reverse_mfma_kernel<Sys, MPGOS_RKCK45, kMfmaPairs>), dim3(grid), dim3(threads), lds, strm, a, A);
+                MPGOS_HIP_TRY(hipGetLastError());
+                MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
+                return MPGOS_OK;
+            }
             const size_t lds = reverse_layout<Sys>(SPB, UPS).total;
             if (lds > ldsMax) { set_error("per-block adjoint: reverse-pass LDS exceeds a workgroup's"); return MPGOS_E_ARG; }
             const int threads = ((SPB * UPS + 63) / 64) * 64;

@@ -77,6 +77,25 @@ def test_ring_adjoint_matches_forward_sensitivities(oracle, alg, kernel):
     assert np.allclose(gU[0, ::ups], gk0[:, 0], rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("alg", [Algorithms.RK4, Algorithms.RKCK45], ids=["rk4", "rkck45"])
+def test_ring_adjoint_mfma_reverse_equals_lane_reverse(oracle, alg):
+    """The dense-contraction reverse pass (MFMA forward and transposed coupling
+    products, stage buffer) returns the lane-per-pair reverse pass's gradients
+    bit for bit; 13 systems (a partial last workgroup of 8), UPS = 64, a
+    checkpoint interval that does not divide the step counts."""
+    spec = _ring(alg, ns=13, ups=64)
+    rs = W.Stream(W.SEED, 304)
+    spec["sPAR"][0] = rs.uniform(spec["ns"], 0.5, 5.0)
+    w = rs.normal(3 * spec["ns"] * spec["ups"]).reshape(3, -1)
+    print(f"ring adjoint lane vs mfma alg={int(alg)}", flush=True)
+    lane = _adjoint(spec, w, K=7, kernel=1)
+    mfma = _adjoint(spec, w, K=7, kernel=2)
+    assert len(np.unique(lane[3]["nacc"])) > 1 or alg == Algorithms.RK4
+    for a, b in zip(lane[:3], mfma[:3]):
+        assert np.isfinite(a).all()
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
 def test_ring_adjoint_checkpoint_interval_and_overflow(oracle):
     spec = _ring(Algorithms.RKCK45, ns=6)
     rs = W.Stream(W.SEED, 301)
