@@ -30,6 +30,9 @@
 // fixed order at the end (deterministic).  Cost per accepted step: one
 // recompute of the step (segment pass) + stage recompute + one transposed
 // coupling product and one VJP per stage.
+// The dense-contraction form (full FP64 matrix; below) runs the coupling
+// products on MFMA and replaces the stage recompute by a stage buffer; it
+// returns the same bits as the lane form.
 #pragma once
 
 #include "pb_device.h"
@@ -41,6 +44,7 @@ template <class Sys>
 struct AdjArgs {
     double* seg;          // [K][UD][NU] (lane form)
     double* stg;          // [K][stages][UD + NC][NU] stage inputs and coupling values (dense-contraction form)
+    double* gSp;          // [NSP][NU] per-unit partials of dL/d(system parameters) (dense-contraction form)
     double* lam;          // in: dL/dx(end), out: dL/dx(start)      [UD][NU]
     double* gU;           // out: dL/d(unit parameters)             [NUP][NU]
     double* gS;           // out: dL/d(system parameters)           [NSP][NS]
@@ -485,142 +489,176 @@ __host__ __device__ inline RevMfmaLayout reverse_mfma_layout(int UPS)
 // Reverse pass in the dense-contraction form (full FP64 coupling matrix, UPS a
 // multiple of 32 up to 256): 8 systems per workgroup, 4 units per lane
 // (2 * UPS threads), every coupling product on v_mfma_f64_16x16x4f64 --
-// M CPT(Y) in the segment recompute, M^T w (cMatT) in the adjoint sweep.
-// The segment recompute keeps each stage's input Y_s and coupling values v_s
-// of every step in the stage buffer
-//   stg[(((j * ST + s) * (UD + NC)) + c) * NU + unit],  j < K,
-// so the adjoint sweep evaluates no forward coupling product again; the
-// coupling factors CPF_s come from the unit's right-hand side at (Y_s, T_s).
-// Sums run in the same order as in the lane form (the accumulation chain of
-// the MFMA instruction is a sequential fused multiply-add in column order),
-// so both forms return the same bits.
+// M CPT(Y) in the segment replay, M^T w (cMatT) in the adjoint sweep.  Round r
+// handles the r-th segment from the end of every system in two launches:
+//   pb_replay_mfma_kernel: restore checkpoint k = nseg - 1 - r and replay the
+//     segment's recorded steps, keeping each stage's input Y_s and coupling
+//     values v_s in the stage buffer
+//       stg[(((j * ST + s) * (UD + NC)) + c) * NU + unit],  j < K;
+//   pb_sweep_mfma_kernel: the adjoint sweep through the segment from the
+//     stage buffer (no forward coupling product again; CPF_s from the unit's
+//     right-hand side at (Y_s, T_s)); dL/dx, the unit-parameter gradient and
+//     the per-unit system-parameter partials carry over between rounds in HBM
+//     (lam, gU, gSp);
+// pb_reverse_finish_kernel sums the partials per system in unit order.  Two
+// kernels instead of one keep the stage-vector registers of the replay and the
+// adjoint registers of the sweep from being live together.  Sums run in the
+// same order as in the lane form (the accumulation chain of the MFMA
+// instruction is a sequential fused multiply-add in column order), so both
+// forms return the same bits.
+template <class Sys, int PAIRS>
+struct MfmaRevCtx {
+    static constexpr int SPB = kMfmaSystems;
+    using Scope = RevScope<Sys>;
+    Scope* sys; double* cptB; double* coup; int* blk;
+    int lanesPerSys, lsys, u0, gsys, k, len, maxlen;
+    bool leader, active;
+    size_t NU;
+    unsigned gu0;
+    Scope* my;
+
+    // lane mapping, system scope and this round's segment of each system
+    // (k = nseg - 1 - r, len steps; maxlen over the workgroup)
+    __device__ __forceinline__ void setup(const Args<Sys, double>& a, unsigned char* lds, int r)
+    {
+        const RevMfmaLayout L = reverse_mfma_layout<Sys>(a.UPS);
+        sys = reinterpret_cast<Scope*>(lds);
+        cptB = reinterpret_cast<double*>(lds + L.cptB);
+        coup = reinterpret_cast<double*>(lds + L.coup);
+        blk = reinterpret_cast<int*>(lds + L.blk);
+        const int tid = threadIdx.x, UPS = a.UPS, NS = a.NS, K = a.recK;
+        lanesPerSys = UPS / PAIRS;
+        lsys = tid / lanesPerSys;                     // < SPB (blockDim.x = SPB * UPS / PAIRS)
+        u0 = tid - lsys * lanesPerSys;
+        leader = u0 == 0;
+        gsys = blockIdx.x * SPB + lsys;
+        active = gsys < NS && gsys < a.nActive;
+        NU = (size_t)NS * UPS;
+        gu0 = (unsigned)gsys * (unsigned)UPS + (unsigned)u0;
+        my = &sys[lsys];
+        for (int i = tid; i < Pos<Sys::NC>::value * UPS * 16; i += blockDim.x) cptB[i] = 0.0;
+        if (tid == 0) blk[1] = 0;
+        if (leader) {
+            my->status = 0; my->k = -1; my->len = 0;
+            if (active) {
+                for (int i = 0; i < Sys::NSP; i++) my->sPAR[i] = a.sPAR[gsys + i * NS];
+                for (int i = 0; i < Sys::NSA; i++) my->sACC[i] = a.sACC[gsys + i * NS];
+                for (int i = 0; i < Sys::NiSA; i++) my->sACCi[i] = a.sACCi[gsys + i * NS];
+                for (int i = 0; i < Sys::NC; i++) my->strength[i] = a.strength[gsys + i * NS];
+                my->nacc = a.nacc[gsys];
+                my->status = a.status[gsys];
+                const bool ok = (my->status & MPGOS_STATUS_CHECKPOINT_OVERFLOW) == 0;
+                const int nseg = ok ? (int)((my->nacc + K - 1) / K) : 0;
+                const int kk = nseg - 1 - r;
+                my->k = kk;
+                my->len = kk >= 0 ? (int)((my->nacc - (long long)kk * K) < K ? (my->nacc - (long long)kk * K) : K) : 0;
+            }
+        }
+        __syncthreads();
+        if (leader) atomicMax(&blk[1], my->len);
+        __syncthreads();
+        maxlen = blk[1];
+        k = my->k;
+        len = my->len;
+    }
+    __device__ __forceinline__ int uid(int p) const { return u0 + p * lanesPerSys; }
+    __device__ __forceinline__ unsigned at(int p) const { return gu0 + (unsigned)(p * lanesPerSys); }
+    // row r of a unit-scope array [rows][NU]: wave-uniform row base, 32-bit lane offset
+    __device__ __forceinline__ double* row(double* base, size_t r) const { return base + r * NU; }
+};
+
 template <class Sys, int ALG, int PAIRS>
-__device__ void pb_reverse_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys>& A)
+__device__ void pb_replay_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys>& A, int r)
 {
     using P = double;
-    constexpr int UD = Sys::UD, NC = Sys::NC, NUP = Sys::NUP, NSP = Sys::NSP;
-    constexpr int NUA = Sys::NUA, NiUA = Sys::NiUA, NSA = Sys::NSA, NiSA = Sys::NiSA;
-    constexpr int NCp = Pos<NC>::value, NUPp = Pos<NUP>::value, NSPp = Pos<NSP>::value;
-    constexpr int ST = ALG == MPGOS_RK4 ? 4 : 6;
-    constexpr int SPB = kMfmaSystems;
-    constexpr int SW = UD + NC;                        // stage-buffer words per unit and stage
+    constexpr int UD = Sys::UD, NC = Sys::NC, NUP = Sys::NUP, NUA = Sys::NUA, NiUA = Sys::NiUA;
+    constexpr int NCp = Pos<NC>::value, NUPp = Pos<NUP>::value;
+    constexpr int ST = ALG == MPGOS_RK4 ? 4 : 6, SW = UD + NC, SPB = kMfmaSystems;
     using math::fma;
-    using Scope = RevScope<Sys>;
     using U = Unit<Sys, P>;
-
     extern __shared__ __align__(16) unsigned char mpgos_pb_lds[];
+    MfmaRevCtx<Sys, PAIRS> C;
+    C.setup(a, mpgos_pb_lds, r);
+    if (C.maxlen == 0) return;                         // workgroup-uniform
     const int UPS = a.UPS, NS = a.NS, K = a.recK;
-    const RevMfmaLayout L = reverse_mfma_layout<Sys>(UPS);
-    Scope* sys = reinterpret_cast<Scope*>(mpgos_pb_lds);
-    P* cptB = reinterpret_cast<P*>(mpgos_pb_lds + L.cptB);
-    P* coup = reinterpret_cast<P*>(mpgos_pb_lds + L.coup);
-    P* red = reinterpret_cast<P*>(mpgos_pb_lds + L.red);             // [SPB][UPS][NSPp]
-    int* blk = reinterpret_cast<int*>(mpgos_pb_lds + L.blk);          // [0] rounds, [1] longest segment
-
-    const int tid = threadIdx.x;
-    const int lanesPerSys = UPS / PAIRS;
-    const int lsys = tid / lanesPerSys;                                 // < SPB (blockDim.x = SPB * UPS / PAIRS)
-    const int u0 = tid - lsys * lanesPerSys;
-    const bool leader = u0 == 0;
-    const int gsys = blockIdx.x * SPB + lsys;
-    const bool active = gsys < NS && gsys < a.nActive;
-    const size_t NU = (size_t)NS * UPS;
-    Scope* my = &sys[lsys];
-    auto uid = [&](int p) { return u0 + p * lanesPerSys; };
-    auto gu = [&](int p) { return (size_t)gsys * UPS + uid(p); };
-    // unit-scope rows [row][NU]: the row offset is wave-uniform (scalar base
-    // address), the unit a 32-bit per-lane offset
-    const unsigned gu0 = (unsigned)gsys * (unsigned)UPS + (unsigned)u0;
-    auto row = [&](double* base, size_t r) { return base + r * NU; };
-    auto at = [&](int p) { return gu0 + (unsigned)(p * lanesPerSys); };
+    const int gsys = C.gsys, lsys = C.lsys, k = C.k, len = C.len;
+    const bool active = C.active;
+    auto* my = C.my;
+    double* cptB = C.cptB;
+    double* coup = C.coup;
 
     int cIndex[NCp];
 #pragma unroll
     for (int i = 0; i < NC; i++) cIndex[i] = a.cIndex[i];
-    for (int i = tid; i < NCp * UPS * 16; i += blockDim.x) cptB[i] = P(0);
-    if (tid < 2) blk[tid] = 0;
-    if (leader) {
-        my->status = 0;
-        my->nseg = 0;
-        if (active) {
-            for (int i = 0; i < NSP; i++) my->sPAR[i] = a.sPAR[gsys + i * NS];
-            for (int i = 0; i < NSA; i++) my->sACC[i] = a.sACC[gsys + i * NS];
-            for (int i = 0; i < NiSA; i++) my->sACCi[i] = a.sACCi[gsys + i * NS];
-            for (int i = 0; i < NC; i++) my->strength[i] = a.strength[gsys + i * NS];
-            my->nacc = a.nacc[gsys];
-            my->status = a.status[gsys];
-            const bool ok = (my->status & MPGOS_STATUS_CHECKPOINT_OVERFLOW) == 0;
-            my->nseg = ok ? (int)((my->nacc + K - 1) / K) : 0;
-        }
-    }
-    P uPAR[PAIRS][NUPp], uACC[PAIRS][Pos<NUA>::value], lam[PAIRS][UD], muU[PAIRS][NUPp], muS[PAIRS][NSPp];
+    P uPAR[PAIRS][NUPp], uACC[PAIRS][Pos<NUA>::value], X[PAIRS][UD];
     int uACCi[PAIRS][Pos<NiUA>::value];
 #pragma unroll
     for (int p = 0; p < PAIRS; p++) {
-        const size_t g = gu(p);
+        const bool ld = active && len > 0;
 #pragma unroll
-        for (int c = 0; c < NUP; c++) { uPAR[p][c] = active ? a.uPAR[g + c * NU] : P(0); muU[p][c] = P(0); }
+        for (int c = 0; c < NUP; c++) uPAR[p][c] = ld ? C.row(a.uPAR, c)[C.at(p)] : P(0);
 #pragma unroll
-        for (int c = 0; c < NUA; c++) uACC[p][c] = active ? a.uACC[g + c * NU] : P(0);
+        for (int c = 0; c < NUA; c++) uACC[p][c] = ld ? C.row(a.uACC, c)[C.at(p)] : P(0);
 #pragma unroll
-        for (int c = 0; c < NiUA; c++) uACCi[p][c] = active ? a.uACCi[g + c * NU] : 0;
+        for (int c = 0; c < NiUA; c++) uACCi[p][c] = ld ? a.uACCi[C.at(p) + c * C.NU] : 0;
 #pragma unroll
-        for (int c = 0; c < UD; c++) lam[p][c] = active ? A.lam[g + c * NU] : P(0);
-#pragma unroll
-        for (int c = 0; c < NSP; c++) muS[p][c] = P(0);
+        for (int c = 0; c < UD; c++) X[p][c] = ld ? C.row(a.recX, (size_t)k * UD + c)[C.at(p)] : P(0);
     }
-    __syncthreads();
-    if (leader) atomicMax(&blk[0], my->nseg);
-    __syncthreads();
-    const int rounds = blk[0];
     P* gPAR = a.gPAR;
     int* gPARi = a.gPARi;
 
     // right-hand sides of the lane's units at (Y, T) plus the coupling terms;
-    // v receives the coupling values (M CPT)_u
+    // v receives the coupling values (M CPT)_u.  The barriers order LDS only,
+    // so the stage-buffer stores stay in flight across them.
     auto rhs = [&](bool run, P (*F)[UD], P (*Y)[UD], const P T, P (*v)[NCp]) {
         P CPF[PAIRS][NCp];
         if (run) {
 #pragma unroll
             for (int p = 0; p < PAIRS; p++) {
                 P cp[NCp];
-                Sys::template OdeFunction<P>(gsys, uid(p), F[p], Y[p], T, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
+                Sys::template OdeFunction<P>(gsys, C.uid(p), F[p], Y[p], T, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
                                              my->sACC, my->sACCi, cp, CPF[p]);
 #pragma unroll
-                for (int i = 0; i < NC; i++) cptB[((size_t)i * UPS + uid(p)) * 16 + lsys] = cp[i];
+                for (int i = 0; i < NC; i++) cptB[((size_t)i * UPS + C.uid(p)) * 16 + lsys] = cp[i];
             }
         }
-        __syncthreads();
-        mfma_contract<NC, 4>(a.cMat, cptB, coup, UPS, SPB);
-        __syncthreads();
+        lds_barrier();
+        mfma_contract<NC>(a.cMat, cptB, coup, UPS, SPB);
+        lds_barrier();
         if (run) {
 #pragma unroll
             for (int p = 0; p < PAIRS; p++)
 #pragma unroll
                 for (int i = 0; i < NC; i++) {
-                    v[p][i] = coup[((size_t)i * UPS + uid(p)) * SPB + lsys];
+                    v[p][i] = coup[((size_t)i * UPS + C.uid(p)) * SPB + lsys];
                     U::add(cIndex[i], my->strength[i] * CPF[p][i], v[p][i], F[p]);
                 }
         }
     };
-    // one forward step of segment step j (the forward pass's arithmetic),
-    // keeping Y_s and v_s in the stage buffer
-    auto forward_step = [&](bool run, P (*X)[UD], const P t, const P h, int j) {
+    for (int j = 0; j < C.maxlen; j++) {
+        const bool run = active && j < len;
+        P t = P(0), h = P(0);
+        if (run) {
+            const size_t n = (size_t)k * K + j;
+            t = a.recT[n * NS + gsys];
+            h = a.recH[n * NS + gsys];
+        }
         auto keep = [&](int s, P (*Y)[UD], P (*v)[NCp]) {
             if (run) {
+                const size_t base = ((size_t)j * ST + s) * SW;
 #pragma unroll
                 for (int p = 0; p < PAIRS; p++) {
-                    const size_t base = ((size_t)j * ST + s) * SW;
 #pragma unroll
-                    for (int c = 0; c < UD; c++) row(A.stg, base + c)[at(p)] = Y[p][c];
+                    for (int c = 0; c < UD; c++) C.row(A.stg, base + c)[C.at(p)] = Y[p][c];
 #pragma unroll
-                    for (int i = 0; i < NC; i++) row(A.stg, base + UD + i)[at(p)] = v[p][i];
+                    for (int i = 0; i < NC; i++) C.row(A.stg, base + UD + i)[C.at(p)] = v[p][i];
                 }
             }
         };
+        // the forward pass's stage arithmetic (pb_body)
         P v[PAIRS][NCp], next[PAIRS][UD], Y[PAIRS][UD];
         if (ALG == MPGOS_RK4) {
-            P k[PAIRS][UD];
+            P k1[PAIRS][UD];
             rhs(run, next, X, t, v);
             keep(0, X, v);
 #pragma unroll
@@ -628,24 +666,24 @@ __device__ void pb_reverse_mfma_body(const Args<Sys, double>& a, const AdjArgs<S
 #pragma unroll
                 for (int c = 0; c < UD; c++) Y[p][c] = fma(next[p][c] * P(0.5), h, X[p][c]);
             const P t2 = t + P(0.5) * h;
-            rhs(run, k, Y, t2, v);
+            rhs(run, k1, Y, t2, v);
             keep(1, Y, v);
 #pragma unroll
             for (int p = 0; p < PAIRS; p++)
 #pragma unroll
-                for (int c = 0; c < UD; c++) { next[p][c] = fma(P(2), k[p][c], next[p][c]); Y[p][c] = fma(k[p][c] * P(0.5), h, X[p][c]); }
-            rhs(run, k, Y, t2, v);
+                for (int c = 0; c < UD; c++) { next[p][c] = fma(P(2), k1[p][c], next[p][c]); Y[p][c] = fma(k1[p][c] * P(0.5), h, X[p][c]); }
+            rhs(run, k1, Y, t2, v);
             keep(2, Y, v);
 #pragma unroll
             for (int p = 0; p < PAIRS; p++)
 #pragma unroll
-                for (int c = 0; c < UD; c++) { next[p][c] = fma(P(2.0), k[p][c], next[p][c]); Y[p][c] = fma(k[p][c], h, X[p][c]); }
-            rhs(run, k, Y, t + h, v);
+                for (int c = 0; c < UD; c++) { next[p][c] = fma(P(2.0), k1[p][c], next[p][c]); Y[p][c] = fma(k1[p][c], h, X[p][c]); }
+            rhs(run, k1, Y, t + h, v);
             keep(3, Y, v);
 #pragma unroll
             for (int p = 0; p < PAIRS; p++)
 #pragma unroll
-                for (int c = 0; c < UD; c++) next[p][c] = fma(h * (next[p][c] + k[p][c]), P(1.0 / 6.0), X[p][c]);
+                for (int c = 0; c < UD; c++) next[p][c] = fma(h * (next[p][c] + k1[p][c]), P(1.0 / 6.0), X[p][c]);
         } else {
             P k1[PAIRS][UD], k2[PAIRS][UD], k3[PAIRS][UD], k4[PAIRS][UD], k5[PAIRS][UD], k6[PAIRS][UD];
             rhs(run, k1, X, t, v);
@@ -698,143 +736,181 @@ __device__ void pb_reverse_mfma_body(const Args<Sys, double>& a, const AdjArgs<S
 #pragma unroll
                 for (int c = 0; c < UD; c++) X[p][c] = next[p][c];
         }
-    };
+    }
+}
 
-    for (int r = 0; r < rounds; r++) {
-        // this round's segment of each system: the r-th from the end
-        if (leader) {
-            const int k = my->nseg - 1 - r;
-            my->k = k;
-            my->len = k >= 0 ? (int)((my->nacc - (long long)k * K) < K ? (my->nacc - (long long)k * K) : K) : 0;
+template <class Sys, int ALG, int PAIRS>
+__device__ void pb_sweep_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys>& A, int r)
+{
+    using P = double;
+    constexpr int UD = Sys::UD, NC = Sys::NC, NUP = Sys::NUP, NSP = Sys::NSP, NUA = Sys::NUA, NiUA = Sys::NiUA;
+    constexpr int NCp = Pos<NC>::value, NUPp = Pos<NUP>::value, NSPp = Pos<NSP>::value;
+    constexpr int ST = ALG == MPGOS_RK4 ? 4 : 6, SW = UD + NC, SPB = kMfmaSystems;
+    using math::fma;
+    extern __shared__ __align__(16) unsigned char mpgos_pb_lds[];
+    MfmaRevCtx<Sys, PAIRS> C;
+    C.setup(a, mpgos_pb_lds, r);
+    if (C.maxlen == 0) return;                         // workgroup-uniform
+    const int UPS = a.UPS, NS = a.NS, K = a.recK;
+    const int gsys = C.gsys, lsys = C.lsys, k = C.k, len = C.len;
+    const bool active = C.active;
+    const bool mine = active && len > 0;               // this round has a segment of this system
+    auto* my = C.my;
+    double* cptB = C.cptB;
+    double* coup = C.coup;
+
+    int cIndex[NCp];
+#pragma unroll
+    for (int i = 0; i < NC; i++) cIndex[i] = a.cIndex[i];
+    // adjoint state carried between rounds; round 0 starts the gradients at zero
+    P uPAR[PAIRS][NUPp], uACC[PAIRS][Pos<NUA>::value], lam[PAIRS][UD], muU[PAIRS][NUPp], muS[PAIRS][NSPp];
+    int uACCi[PAIRS][Pos<NiUA>::value];
+#pragma unroll
+    for (int p = 0; p < PAIRS; p++) {
+#pragma unroll
+        for (int c = 0; c < NUP; c++) {
+            uPAR[p][c] = mine ? C.row(a.uPAR, c)[C.at(p)] : P(0);
+            muU[p][c] = (mine && r > 0) ? C.row(A.gU, c)[C.at(p)] : P(0);
         }
-        if (tid == 0) blk[1] = 0;
-        __syncthreads();
-        if (leader) atomicMax(&blk[1], my->len);
-        __syncthreads();
-        const int maxlen = blk[1];
-        const int k = my->k, len = my->len;
-        P X[PAIRS][UD];
 #pragma unroll
-        for (int p = 0; p < PAIRS; p++)
+        for (int c = 0; c < NUA; c++) uACC[p][c] = mine ? C.row(a.uACC, c)[C.at(p)] : P(0);
 #pragma unroll
-            for (int c = 0; c < UD; c++) X[p][c] = (active && len > 0) ? row(a.recX, (size_t)k * UD + c)[at(p)] : P(0);
-        // 1. recompute the segment, keeping every stage's input and coupling values
-        for (int j = 0; j < maxlen; j++) {
-            const bool run = active && j < len;
-            P t = P(0), h = P(0);
-            if (run) {
-                const size_t n = (size_t)k * K + j;
-                t = a.recT[n * NS + gsys];
-                h = a.recH[n * NS + gsys];
-            }
-            forward_step(run, X, t, h, j);
+        for (int c = 0; c < NiUA; c++) uACCi[p][c] = mine ? a.uACCi[C.at(p) + c * C.NU] : 0;
+#pragma unroll
+        for (int c = 0; c < UD; c++) lam[p][c] = mine ? C.row(A.lam, c)[C.at(p)] : P(0);
+#pragma unroll
+        for (int c = 0; c < NSP; c++) muS[p][c] = (mine && r > 0) ? C.row(A.gSp, c)[C.at(p)] : P(0);
+    }
+    P* gPAR = a.gPAR;
+    int* gPARi = a.gPARi;
+
+    for (int j = C.maxlen - 1; j >= 0; j--) {
+        const bool run = active && j < len;
+        P t = P(0), h = P(0);
+        if (run) {
+            const size_t n = (size_t)k * K + j;
+            t = a.recT[n * NS + gsys];
+            h = a.recH[n * NS + gsys];
         }
-        // 2. adjoint sweep through the segment
-        for (int j = maxlen - 1; j >= 0; j--) {
-            const bool run = active && j < len;
-            P t = P(0), h = P(0);
+        P kb[ST][PAIRS][UD];
+#pragma unroll
+        for (int s = 0; s < ST; s++)
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++)
+#pragma unroll
+                for (int c = 0; c < UD; c++) kb[s][p][c] = h * rk_b<ALG>(s) * lam[p][c];
+#pragma unroll
+        for (int s = ST - 1; s >= 0; s--) {
+            const P Ts = rk_stage_time<ALG>(s, t, h);
+            const size_t base = ((size_t)j * ST + s) * SW;
+            P kbCPF[PAIRS][NCp];
             if (run) {
-                const size_t n = (size_t)k * K + j;
-                t = a.recT[n * NS + gsys];
-                h = a.recH[n * NS + gsys];
-            }
-            P kb[ST][PAIRS][UD];
-#pragma unroll
-            for (int s = 0; s < ST; s++)
-#pragma unroll
-                for (int p = 0; p < PAIRS; p++)
-#pragma unroll
-                    for (int c = 0; c < UD; c++) kb[s][p][c] = h * rk_b<ALG>(s) * lam[p][c];
-#pragma unroll
-            for (int s = ST - 1; s >= 0; s--) {
-                const P Ts = rk_stage_time<ALG>(s, t, h);
-                const size_t base = ((size_t)j * ST + s) * SW;
-                P kbCPF[PAIRS][NCp];
-                if (run) {
-#pragma unroll
-                    for (int p = 0; p < PAIRS; p++) {
-                        P Y[UD], F[UD], cp[NCp], cpf[NCp];
-#pragma unroll
-                        for (int c = 0; c < UD; c++) Y[c] = row(A.stg, base + c)[at(p)];
-                        Sys::template OdeFunction<P>(gsys, uid(p), F, Y, Ts, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
-                                                     my->sACC, my->sACCi, cp, cpf);
-#pragma unroll
-                        for (int i = 0; i < NC; i++) {
-                            P g = P(0);
-#pragma unroll
-                            for (int c = 0; c < UD; c++)
-                                if (c == cIndex[i]) g = kb[s][p][c];
-                            cptB[((size_t)i * UPS + uid(p)) * 16 + lsys] = my->strength[i] * cpf[i] * g;
-                            kbCPF[p][i] = my->strength[i] * row(A.stg, base + UD + i)[at(p)] * g;
-                        }
-                    }
-                }
-                __syncthreads();
-                mfma_contract<NC, 4>(A.cMatT, cptB, coup, UPS, SPB);
-                __syncthreads();
 #pragma unroll
                 for (int p = 0; p < PAIRS; p++) {
-                    P gy[UD];
-                    if (run) {
-                        P Y[UD], kbCPT[NCp], gU[NUPp], gS[NSPp];
+                    P Y[UD], F[UD], cp[NCp], cpf[NCp];
 #pragma unroll
-                        for (int c = 0; c < UD; c++) Y[c] = row(A.stg, base + c)[at(p)];   // reloaded rather than held across the contraction
+                    for (int c = 0; c < UD; c++) Y[c] = C.row(A.stg, base + c)[C.at(p)];
+                    Sys::template OdeFunction<P>(gsys, C.uid(p), F, Y, Ts, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
+                                                 my->sACC, my->sACCi, cp, cpf);
 #pragma unroll
-                        for (int i = 0; i < NC; i++) kbCPT[i] = coup[((size_t)i * UPS + uid(p)) * SPB + lsys];
-                        Sys::template OdeVJP<P>(gsys, uid(p), kb[s][p], kbCPT, kbCPF[p], Y, Ts, uPAR[p], my->sPAR, gPAR, gPARi,
-                                                uACC[p], uACCi[p], my->sACC, my->sACCi, gy, gU, gS);
+                    for (int i = 0; i < NC; i++) {
+                        P g = P(0);
 #pragma unroll
-                        for (int c = 0; c < NUP; c++) muU[p][c] += gU[c];
-#pragma unroll
-                        for (int c = 0; c < NSP; c++) muS[p][c] += gS[c];
-                    } else {
-#pragma unroll
-                        for (int c = 0; c < UD; c++) gy[c] = P(0);
+                        for (int c = 0; c < UD; c++)
+                            if (c == cIndex[i]) g = kb[s][p][c];
+                        cptB[((size_t)i * UPS + C.uid(p)) * 16 + lsys] = my->strength[i] * cpf[i] * g;
+                        kbCPF[p][i] = my->strength[i] * C.row(A.stg, base + UD + i)[C.at(p)] * g;
                     }
-#pragma unroll
-                    for (int c = 0; c < UD; c++) lam[p][c] += gy[c];
-#pragma unroll
-                    for (int l = 0; l < ST; l++)
-                        if (l < s && rk_a<ALG>(s, l) != P(0)) {
-#pragma unroll
-                            for (int c = 0; c < UD; c++) kb[l][p][c] = fma(h * rk_a<ALG>(s, l), gy[c], kb[l][p][c]);
-                        }
                 }
             }
+            __syncthreads();
+            mfma_contract<NC, 4>(A.cMatT, cptB, coup, UPS, SPB);
+            __syncthreads();
+#pragma unroll
+            for (int p = 0; p < PAIRS; p++) {
+                P gy[UD];
+                if (run) {
+                    P Y[UD], kbCPT[NCp], gU[NUPp], gS[NSPp];
+#pragma unroll
+                    for (int c = 0; c < UD; c++) Y[c] = C.row(A.stg, base + c)[C.at(p)];
+#pragma unroll
+                    for (int i = 0; i < NC; i++) kbCPT[i] = coup[((size_t)i * UPS + C.uid(p)) * SPB + lsys];
+                    Sys::template OdeVJP<P>(gsys, C.uid(p), kb[s][p], kbCPT, kbCPF[p], Y, Ts, uPAR[p], my->sPAR, gPAR, gPARi,
+                                            uACC[p], uACCi[p], my->sACC, my->sACCi, gy, gU, gS);
+#pragma unroll
+                    for (int c = 0; c < NUP; c++) muU[p][c] += gU[c];
+#pragma unroll
+                    for (int c = 0; c < NSP; c++) muS[p][c] += gS[c];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < UD; c++) gy[c] = P(0);
+                }
+#pragma unroll
+                for (int c = 0; c < UD; c++) lam[p][c] += gy[c];
+#pragma unroll
+                for (int l = 0; l < ST; l++)
+                    if (l < s && rk_a<ALG>(s, l) != P(0)) {
+#pragma unroll
+                        for (int c = 0; c < UD; c++) kb[l][p][c] = fma(h * rk_a<ALG>(s, l), gy[c], kb[l][p][c]);
+                    }
+            }
         }
-        __syncthreads();
     }
-
-    // outputs: dL/dx(start), dL/d(unit parameters), dL/d(system parameters)
-    const bool bad = active && (my->status & MPGOS_STATUS_CHECKPOINT_OVERFLOW) != 0;
-    if (active) {
-        const P nan = __builtin_nan("");
+    if (mine) {
 #pragma unroll
         for (int p = 0; p < PAIRS; p++) {
 #pragma unroll
-            for (int c = 0; c < UD; c++) A.lam[gu(p) + c * NU] = bad ? nan : lam[p][c];
+            for (int c = 0; c < UD; c++) C.row(A.lam, c)[C.at(p)] = lam[p][c];
 #pragma unroll
-            for (int c = 0; c < NUP; c++) A.gU[gu(p) + c * NU] = bad ? nan : muU[p][c];
-        }
-    }
+            for (int c = 0; c < NUP; c++) C.row(A.gU, c)[C.at(p)] = muU[p][c];
 #pragma unroll
-    for (int p = 0; p < PAIRS; p++)
-#pragma unroll
-        for (int c = 0; c < NSP; c++) red[((size_t)lsys * UPS + uid(p)) * NSPp + c] = muS[p][c];
-    __syncthreads();
-    if (leader && active) {
-        for (int c = 0; c < NSP; c++) {
-            P s = P(0);
-            for (int v = 0; v < UPS; v++) s += red[((size_t)lsys * UPS + v) * NSPp + c];
-            A.gS[gsys + (size_t)c * NS] = bad ? __builtin_nan("") : s;
+            for (int c = 0; c < NSP; c++) C.row(A.gSp, c)[C.at(p)] = muS[p][c];
         }
     }
 }
 
 template <class Sys, int ALG, int PAIRS>
-__global__ void __launch_bounds__(kMfmaSystems * 256 / PAIRS) pb_reverse_mfma_kernel(Args<Sys, double> a, AdjArgs<Sys> A)
+__global__ void __launch_bounds__(kMfmaSystems * 256 / PAIRS) pb_replay_mfma_kernel(Args<Sys, double> a, AdjArgs<Sys> A, int r)
 {
-    pb_reverse_mfma_body<Sys, ALG, PAIRS>(a, A);
+    pb_replay_mfma_body<Sys, ALG, PAIRS>(a, A, r);
+}
+
+template <class Sys, int ALG, int PAIRS>
+__global__ void __launch_bounds__(kMfmaSystems * 256 / PAIRS) pb_sweep_mfma_kernel(Args<Sys, double> a, AdjArgs<Sys> A, int r)
+{
+    pb_sweep_mfma_body<Sys, ALG, PAIRS>(a, A, r);
+}
+
+// Outputs of the dense-contraction reverse pass, one thread per unit: systems
+// without accepted steps get zero gradients (dL/dx unchanged), systems whose
+// record overflowed NaN; dL/d(system parameters) sums the per-unit partials in
+// unit order (thread 0 of the system's units).
+template <class Sys>
+__global__ void pb_reverse_finish_kernel(Args<Sys, double> a, AdjArgs<Sys> A)
+{
+    constexpr int UD = Sys::UD, NUP = Sys::NUP, NSP = Sys::NSP;
+    const size_t NU = (size_t)a.NS * a.UPS;
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NU) return;
+    const int gsys = (int)(g / a.UPS), u = (int)(g - (size_t)gsys * a.UPS);
+    if (gsys >= a.nActive) return;
+    const bool bad = (a.status[gsys] & MPGOS_STATUS_CHECKPOINT_OVERFLOW) != 0;
+    const bool none = a.nacc[gsys] == 0;
+    const double nan = __builtin_nan("");
+    if (bad) {
+        for (int c = 0; c < UD; c++) A.lam[g + c * NU] = nan;
+        for (int c = 0; c < NUP; c++) A.gU[g + c * NU] = nan;
+    } else if (none) {
+        for (int c = 0; c < NUP; c++) A.gU[g + c * NU] = 0.0;
+    }
+    if (u == 0) {
+        for (int c = 0; c < NSP; c++) {
+            double s = 0.0;
+            if (!none)
+                for (int v = 0; v < a.UPS; v++) s += A.gSp[(size_t)c * NU + g + v];
+            A.gS[gsys + (size_t)c * a.NS] = bad ? nan : s;
+        }
+    }
 }
 
 } // namespace pb

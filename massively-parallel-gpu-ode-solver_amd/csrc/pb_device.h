@@ -54,6 +54,18 @@ namespace pb {
 
 template <int N> struct Pos { static constexpr int value = N > 0 ? N : 1; };
 
+// Workgroup barrier that orders LDS only: waits for the wave's own LDS
+// operations (lgkmcnt) and not for its global loads and stores, which stay in
+// flight across the barrier (__syncthreads also drains vmcnt).  For barriers
+// whose only purpose is LDS communication; global data exchanged between lanes
+// still needs __syncthreads.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 constexpr int kMaxBlock = 1024;
 constexpr int kMfmaSystems = 8;          // systems per workgroup of the dense-contraction form
 constexpr int kMfmaPairs = 4;            // default pairs per lane of the dense-contraction form (2 * UPS threads)

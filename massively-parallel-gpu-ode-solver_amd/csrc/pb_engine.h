@@ -82,15 +82,15 @@ public:
     static constexpr bool kAdjoint = std::is_same<P, double>::value && HasVjp<Sys>::value;
     Buffer<P> lamB, gUB, gSB;           // dL/dx, dL/d(unit parameters), dL/d(system parameters)
     double* recX = nullptr; double* recT = nullptr; double* recH = nullptr; double* seg = nullptr; double* cMatT = nullptr;
-    double* stg = nullptr;
-    size_t recXn = 0, recTn = 0, recHn = 0, segn = 0, cMatTn = 0, stgn = 0;
+    double* stg = nullptr; double* gSp = nullptr;
+    size_t recXn = 0, recTn = 0, recHn = 0, segn = 0, cMatTn = 0, stgn = 0, gSpn = 0;
     int adjK = 0, adjCap = 0, adjStepCap = 0;
 
     ~Solver() override
     {
         if (strm) (void)hipStreamSynchronize(strm);
         lamB.release(); gUB.release(); gSB.release();
-        for (double* q : {recX, recT, recH, seg, cMatT, stg})
+        for (double* q : {recX, recT, recH, seg, cMatT, stg, gSp})
             if (q) (void)hipFree(q);
         for (auto* b : {&X, &uPAR, &uACC, &TD, &T, &sPAR, &sACC, &strength, &DOt, &DOx, &gPAR, &cMat}) b->release();
         for (auto* b : {&uACCi, &sACCi, &DOidx, &gPARi, &cIndex, &status}) b->release();
@@ -569,17 +569,40 @@ public:
             Args<Sys, double> a = make_args();
             a.recX = recX; a.recT = recT; a.recH = recH; a.recK = adjK; a.recCkCap = adjCap; a.recStepCap = adjStepCap;
             AdjArgs<Sys> A;
-            A.seg = seg; A.stg = stg; A.lam = lamB.d; A.gU = gUB.d; A.gS = gSB.d; A.cMatT = cMatT;
+            A.seg = seg; A.stg = stg; A.gSp = nullptr; A.lam = lamB.d; A.gU = gUB.d; A.gS = gSB.d; A.cMatT = cMatT;
             lastForm = cpl;
             if (cpl == kMfmaCoupling) {
+                // one replay and one sweep launch per round; rounds = the largest
+                // segment count of a system whose record did not overflow
                 a.SPB = kMfmaSystems;
                 const size_t lds = reverse_mfma_layout<Sys>(UPS).total;
                 if (lds > ldsMax) { set_error("per-block adjoint: reverse-pass LDS exceeds a workgroup's"); return MPGOS_E_ARG; }
+                if ((rc = grow_device(gSp, gSpn, (size_t)(NSP > 0 ? NSP : 1) * NU, "system-parameter partials"))) return rc;
+                A.gSp = gSp;
+                std::vector<long long> nh((size_t)NS);
+                std::vector<int> sh((size_t)NS);
+                MPGOS_HIP_TRY(hipMemcpy(nh.data(), nacc.d, nh.size() * sizeof(long long), hipMemcpyDeviceToHost));
+                MPGOS_HIP_TRY(hipMemcpy(sh.data(), status.d, sh.size() * sizeof(int), hipMemcpyDeviceToHost));
+                int rounds = 0;
+                for (int i = 0; i < nActive && i < NS; i++)
+                    if ((sh[i] & MPGOS_STATUS_CHECKPOINT_OVERFLOW) == 0) {
+                        const long long nseg = (nh[i] + adjK - 1) / adjK;
+                        if (nseg > rounds) rounds = (int)nseg;
+                    }
                 const int threads = kMfmaSystems * UPS / kMfmaPairs;
                 const int grid = NS / kMfmaSystems + (NS % kMfmaSystems == 0 ? 0 : 1);
+                const int fgrid = (int)((NU + 255) / 256);
                 MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
-                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_reverse_mfma_kernel<Sys, MPGOS_RK4, kMfmaPairs>), dim3(grid), dim3(threads), lds, strm, a, A);
-                else hipLaunchKernelGGL((pb_reverse_mfma_kernel<Sys, MPGOS_RKCK45, kMfmaPairs>), dim3(grid), dim3(threads), lds, strm, a, A);
+                for (int r = 0; r < rounds; r++) {
+                    if (algorithm == MPGOS_RK4) {
+                        hipLaunchKernelGGL((pb_replay_mfma_kernel<Sys, MPGOS_RK4, kMfmaPairs>), dim3(grid), dim3(threads), lds, strm, a, A, r);
+                        hipLaunchKernelGGL((pb_sweep_mfma_kernel<Sys, MPGOS_RK4, kMfmaPairs>), dim3(grid), dim3(threads), lds, strm, a, A, r);
+                    } else {
+                        hipLaunchKernelGGL((pb_replay_mfma_kernel<Sys, MPGOS_RKCK45, kMfmaPairs>), dim3(grid), dim3(threads), lds, strm, a, A, r);
+                        hipLaunchKernelGGL((pb_sweep_mfma_kernel<Sys, MPGOS_RKCK45, kMfmaPairs>), dim3(grid), dim3(threads), lds, strm, a, A, r);
+                    }
+                }
+                hipLaunchKernelGGL((pb_reverse_finish_kernel<Sys>), dim3(fgrid), dim3(256), 0, strm, a, A);
                 MPGOS_HIP_TRY(hipGetLastError());
                 MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
                 return MPGOS_OK;
