@@ -289,13 +289,23 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
                 j++;
             }
         }
-        for (int jj = j - 1; jj >= 0; jj--) {
+        // walk back through the segment; step jj - 1's record is loaded while
+        // step jj is differentiated (hides the memory latency)
+        double xq[SD], tq = 0.0, hq = 0.0;
+        auto load = [&](int jj) {
             const size_t sb = (size_t)jj * (SD + 2);
+#pragma unroll
+            for (int c = 0; c < SD; c++) xq[c] = A.seg[(sb + c) * NT + tid];
+            tq = A.seg[(sb + SD) * NT + tid];
+            hq = A.seg[(sb + SD + 1) * NT + tid];
+        };
+        if (j > 0) load(j - 1);
+        for (int jj = j - 1; jj >= 0; jj--) {
             double x[SD];
 #pragma unroll
-            for (int c = 0; c < SD; c++) x[c] = A.seg[(sb + c) * NT + tid];
-            const double tn = A.seg[(sb + SD) * NT + tid];
-            const double hn = A.seg[(sb + SD + 1) * NT + tid];
+            for (int c = 0; c < SD; c++) x[c] = xq[c];
+            const double tn = tq, hn = hq;
+            if (jj > 0) load(jj - 1);
             if (ALG == MPGOS_RK4)
                 reverse_rk4<Sys>(tid, NT, x, tn, hn, L.cPAR, sPAR, sPARi, L.ACC, L.ACCi, lam, mu);
             else
