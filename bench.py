@@ -230,26 +230,31 @@ def adjoint(args, mpgos, W, pyoracle):
     spec = W.config3_duffing_adjoint(nt=args.nt or 65536, algorithm=alg)
     nt = spec["nt"]
     K = args.checkpoint_interval
+    ktable = None
     s = mpgos.ProblemSolver(spec["system"], nt, alg)
     try:
         W.load(s, spec)
         snap = DeviceSnapshot(s)
         lam = np.zeros((3, nt))
         lam[2] = 1.0                                    # dL/dq(H): d/dq of the quadrature state
+        s.Solve()
+        s.SynchroniseFromDeviceToHost(V.All)
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
+        out = W.read(s)
+        steps_needed = int(out["nacc"].max())
+        snap.restore()
+        snap.sync()
+        if K == 0:                                      # checkpoint interval from the autotuner
+            from mpgos.tuning import autotune_adjoint
+            K, ktable = autotune_adjoint(s, lam, steps_needed, candidates=(16, 32, 64, 128, 256), log=log)
+        cap = steps_needed // K + 2
         fwd, rec, rev = [], [], []
-        steps_needed = None
         for i in range(args.warmup + args.steps):
             snap.restore()
             s.Solve()
             snap.sync()
             f_ms = s.last_solve_ms()
-            if steps_needed is None:
-                s.SynchroniseFromDeviceToHost(V.All)
-                s.InsertSynchronisationPoint()
-                s.SynchroniseSolver()
-                out = W.read(s)
-                steps_needed = int(out["nacc"].max())
-                cap = steps_needed // K + 2
             snap.restore()
             s.adjoint_record(K, cap)
             snap.sync()
@@ -285,7 +290,10 @@ def adjoint(args, mpgos, W, pyoracle):
                           "flops_per_attempted_step": fpa, "kernel_ms": f},
                          cpu, None)
     line["adjoint"] = {"forward_ms": f, "record_ms": r, "reverse_ms": b, "adjoint_over_forward": (r + b) / f,
-                       "reverse_over_forward": b / f}
+                       "reverse_over_forward": b / f, "checkpoint_interval": K,
+                       "checkpoint_interval_from": "autotuner" if ktable else "command line"}
+    if ktable:
+        line["adjoint"]["checkpoint_autotune"] = ktable
     return line
 
 
@@ -364,6 +372,7 @@ def ring_adjoint(args, W, ns, U, kernel):
     from mpgos.perblock import PB, PBOption, PBProblemSolver
     spec = W.config4_ring(ns=ns, ups=U, periods=4, system="ring_duffing_q")
     K = args.checkpoint_interval
+    ktable = None
     s = PBProblemSolver("ring_duffing_q", ns, U, spec["algorithm"])
     fwd, rec, rev = [], [], []
     try:
@@ -373,6 +382,22 @@ def ring_adjoint(args, W, ns, U, kernel):
         nmax = int(W.read_pb(s)["nacc"].max())
         lam = np.zeros((3, ns * U))
         lam[2] = 1.0 / (2.0 * np.pi * U)                   # d(mean over units of q(end))/dq / period
+        if K == 0:                                          # checkpoint interval from the autotuner
+            from mpgos.tuning import autotune_checkpoint_interval
+
+            def record(k):
+                W.load_pb(s, spec)
+                s.adjoint_record(k, nmax // k + 2, nmax + 2)
+                s.SynchroniseDevice()
+                return s.last_solve_ms()
+
+            def reverse():
+                s.host_array(PB.AdjointState)[...] = lam
+                s.SynchroniseFromHostToDevice(PB.AdjointState)
+                s.adjoint_reverse()
+                s.SynchroniseDevice()
+                return s.last_solve_ms()
+            K, ktable = autotune_checkpoint_interval(record, reverse, candidates=(8, 16, 32, 64), log=log)
         for i in range(args.warmup + args.steps):
             W.load_pb(s, spec)
             s.Solve()
@@ -398,9 +423,10 @@ def ring_adjoint(args, W, ns, U, kernel):
         s.close()
     f, r, b = float(np.mean(fwd)), float(np.mean(rec)), float(np.mean(rev))
     return {"system": "ring_duffing_q (unit state x, v, q with q' = x^2)", "gradient": "dL/dB per system",
-            "checkpoint_interval": K, "forward_ms": f, "record_ms": r, "reverse_ms": b,
+            "checkpoint_interval": K, "checkpoint_interval_from": "autotuner" if ktable else "command line",
+            "forward_ms": f, "record_ms": r, "reverse_ms": b,
             "adjoint_over_forward": (r + b) / f, "reverse_over_forward": b / f,
-            "gradient_finite": bool(np.isfinite(gB).all())}
+            "gradient_finite": bool(np.isfinite(gB).all()), **({"checkpoint_autotune": ktable} if ktable else {})}
 
 
 # --------------------------------------------------------------------------- common
@@ -465,7 +491,8 @@ def main():
     ap.add_argument("--form", default="auto", help="auto | plain | persistent (per-thread); auto | mfma | lane (ring)")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "f32k"])
     ap.add_argument("--algorithm", default="rk4", choices=["rk4", "rkck45"], help="duffing_adjoint only")
-    ap.add_argument("--checkpoint-interval", type=int, default=64)
+    ap.add_argument("--checkpoint-interval", type=int, default=0,
+                    help="adjoint checkpoint interval K; 0 = picked by the autotuner")
     ap.add_argument("--slice-steps", type=int, default=2000000,
                     help="duffing_uneven: attempted steps per instance per launch of the time-sliced solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")

@@ -13,6 +13,9 @@ Results are bit-identical across candidates (the arithmetic of an instance
 does not depend on its lane), so the choice is purely about speed.  The
 winning configuration can be cached in a JSON file keyed by system,
 algorithm, precision and instance count.
+autotune_checkpoint_interval() does the same for the checkpoint interval K of
+the discrete adjoint (record + reverse time; the gradients do not depend on
+K), for per-thread and per-block solvers alike.
 """
 import ctypes
 import itertools
@@ -152,3 +155,59 @@ def autotune(solver, candidates=None, repeats=2, cache=None, key=None, log=None)
         with open(cache, "w") as f:
             json.dump(known, f, indent=1)
     return best, table
+
+
+def autotune_checkpoint_interval(run_record, run_reverse, candidates=(8, 16, 32, 64, 128, 256), repeats=1, log=None):
+    """Checkpoint interval K of the discrete adjoint with the smallest
+    record + reverse time.  `run_record(K)` performs a record pass from the
+    solver's inputs and returns its kernel time in ms, `run_reverse()` the
+    reverse pass (with dL/dx(end) already in place) and returns its time.  The
+    gradients are identical bit for bit for every K, so the choice only trades
+    recomputation and memory for speed.  Returns (K, table)."""
+    table = []
+    for K in candidates:
+        rec, rev = [], []
+        try:
+            for _ in range(max(1, repeats)):
+                rec.append(run_record(K))
+                rev.append(run_reverse())
+        except Exception as e:  # e.g. an allocation the device cannot hold
+            if log:
+                log(f"autotune K: skip {K}: {e}")
+            continue
+        row = {"K": K, "record_ms": float(np.mean(rec)), "reverse_ms": float(np.mean(rev))}
+        row["total_ms"] = row["record_ms"] + row["reverse_ms"]
+        table.append(row)
+        if log:
+            log(f"autotune K={K}: record {row['record_ms']:.2f} ms reverse {row['reverse_ms']:.2f} ms")
+    if not table:
+        raise RuntimeError("autotune_checkpoint_interval: no candidate ran")
+    return min(table, key=lambda r: r["total_ms"])["K"], table
+
+
+def autotune_adjoint(solver, lam_T, max_steps, candidates=(8, 16, 32, 64, 128, 256), repeats=1, log=None):
+    """autotune_checkpoint_interval for a per-thread ProblemSolver: every record
+    pass starts from the inputs held when called (device snapshot); `lam_T` is
+    dL/dx(T) (SD x NT) and `max_steps` the largest accepted-step count of an
+    instance (the checkpoint capacity is sized from it).  Leaves the solver's
+    inputs restored.  Returns (K, table)."""
+    snap = DeviceSnapshot(solver)
+
+    def record(K):
+        snap.restore()
+        solver.adjoint_record(K, max_steps // K + 2)
+        snap.sync()
+        return solver.last_solve_ms()
+
+    def reverse():
+        solver.host_array(V.AdjointState)[...] = lam_T
+        solver.SynchroniseFromHostToDevice(V.AdjointState)
+        solver.adjoint_reverse()
+        snap.sync()
+        return solver.last_solve_ms()
+    try:
+        return autotune_checkpoint_interval(record, reverse, candidates, repeats, log)
+    finally:
+        snap.restore()
+        snap.sync()
+        snap.close()

@@ -144,3 +144,31 @@ def test_torch_autograd_two_segment_loss(gpu, oracle, algorithm):
     dL_dx0 = np.einsum("an,apn->pn", dq_dxv, J1[:2, :2]) / two_pi
     assert rel(k.grad.numpy(), dL_dp[0]) < 1e-6 and rel(B.grad.numpy(), dL_dp[1]) < 1e-6
     assert rel(x0.grad.numpy(), dL_dx0[0]) < 1e-6 and rel(v0.grad.numpy(), dL_dx0[1]) < 1e-6
+
+
+def test_checkpoint_interval_autotuner(gpu, oracle):
+    """The autotuner times record + reverse for each candidate K, leaves the
+    solver's inputs as they were, and the gradients with its K equal those
+    with K = 1 bit for bit."""
+    from mpgos.tuning import autotune_adjoint
+    spec = W.config3_duffing_adjoint(nt=256, algorithm=Algorithms.RKCK45)
+    spec["TD"][1] = 2 * (2 * W.PI)
+    lam_T = np.random.default_rng(5).normal(size=(3, 256))
+    base = gpu_adjoint(spec, lam_T, K=1)
+    s = ProblemSolver(spec["system"], spec["nt"], spec["algorithm"])
+    try:
+        W.load(s, spec)
+        x0 = spec["X"].copy()
+        nmax = int(base[2]["nacc"].max())
+        K, table = autotune_adjoint(s, lam_T, nmax, candidates=(4, 16, 64))
+        assert K in (4, 16, 64) and len(table) == 3
+        assert all(r["record_ms"] > 0 and r["reverse_ms"] > 0 for r in table)
+        s.SynchroniseFromDeviceToHost(V.ActualState)
+        s.InsertSynchronisationPoint()
+        s.SynchroniseSolver()
+        assert np.array_equal(s.host_array(V.ActualState), x0)      # inputs restored
+        lam0, gp = s.adjoint(lam_T, K, nmax // K + 2)
+    finally:
+        s.close()
+    assert np.array_equal(lam0.view(np.uint64), base[0].view(np.uint64))
+    assert np.array_equal(gp.view(np.uint64), base[1].view(np.uint64))
