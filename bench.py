@@ -168,7 +168,10 @@ def per_thread(args, mpgos, W, pyoracle):
     if args.form in ("auto", "plain"):
         forms["plain"] = time_form(mpgos, spec, {"mode": 0, "block": 64}, args.steps, args.warmup, prec, sl)
     if args.form in ("auto", "persistent"):
-        forms["persistent"] = time_form(mpgos, spec, {"mode": 1, "block": 256, "round": 32}, args.steps, args.warmup, prec, sl)
+        pcfg = {"mode": 1, "block": 256, "round": 32}
+        if sl and args.solo:       # longest interval first, the longest instances alone in one-wave workgroups
+            pcfg = {"mode": 1, "block": 64, "round": 256, "waves_per_simd": 1, "order": 1, "solo": args.solo}
+        forms["persistent"] = time_form(mpgos, spec, pcfg, args.steps, args.warmup, prec, sl)
     if args.form == "auto" and not args.no_tune and not sl:
         best_cfg = tune(mpgos, spec, prec)
         forms["autotuned"] = time_form(mpgos, spec, best_cfg, args.steps, args.warmup, prec)
@@ -495,6 +498,8 @@ def main():
                     help="adjoint checkpoint interval K; 0 = picked by the autotuner")
     ap.add_argument("--slice-steps", type=int, default=2000000,
                     help="duffing_uneven: attempted steps per instance per launch of the time-sliced solve")
+    ap.add_argument("--solo", type=int, default=0,
+                    help="duffing_uneven, persistent form: longest instances run alone in a workgroup")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--no-adjoint", action="store_true", help="ring: skip the per-block adjoint timing")

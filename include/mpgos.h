@@ -103,7 +103,19 @@ enum {
     MPGOS_PT_EVENT_LOG_CAPACITY,   /* hits logged per instance (allocates EVENT_LOG), default 0 */
     MPGOS_PT_LAUNCH_VARIANT,       /* register-budget variant of the kernel (0 = default, 1..3 = waves-per-EU 2,4,8) */
     MPGOS_PT_PERSISTENT_ROUND_STEPS, /* persistent form: attempted steps between lane compactions (default 32) */
+    MPGOS_PT_QUEUE_ORDER,          /* persistent form: order instances are taken from the work queue (MPGOS_QUEUE_*) */
+    MPGOS_PT_PERSISTENT_SOLO_INSTANCES, /* persistent form: the first N instances of the queue each run alone in a
+                                           workgroup's first lane (no lane divergence on the longest serial chains);
+                                           at most one per workgroup, default 0 */
     MPGOS_PT_NUM_OPTIONS
+};
+
+/* MPGOS_PT_QUEUE_ORDER values.  The order only schedules: results are
+ * bit-identical for every order. */
+enum {
+    MPGOS_QUEUE_INDEX = 0,           /* instance index order (default) */
+    MPGOS_QUEUE_LONGEST_FIRST = 1    /* longest integration interval TimeDomain[1] - TimeDomain[0] first
+                                        (longest-processing-time-first when steps scale with the interval) */
 };
 
 /* MPGOS_PT_KERNEL_MODE values */

@@ -54,17 +54,20 @@ struct Args {
     int* DOidx; P* DOt; P* DOx;
     // per-instance outputs
     long long* nacc; long long* nrej; int* status; int* nev; P* evlog; int* evlogidx;
-    unsigned int* queue;      // persistent form: [0] = next list entry; [1] = suspended count of a slice
+    unsigned int* queue;      // persistent form: [0] = next list entry past the solo ones; [1] = suspended count
+                              // of a slice; [2] = next solo entry
     // time-sliced solve: the instances to work on are list[0, listLen) (the
     // identity 0..nActive-1 when list is null); with slice > 0 an instance
     // that has attempted `slice` steps in this launch is suspended: its lane
-    // state goes to saved[w * NT + id] (32-bit words), phase[id] = 1, and its id
-    // is appended to listOut (count in queue[1]); phase[id] = 2 once finished
+    // state goes to its record saved[id * W .. id * W + W) (W 32-bit words,
+    // 16-byte aligned), phase[id] = 1, and its id is appended to listOut
+    // (count in queue[1]); phase[id] = 2 once finished
     const int* list; int* listOut; uint32_t* saved; int* phase;
     int listLen, slice;
     int NT, nActive, NDO, evcap, preferShared;
     int doFreq;
     int roundSteps;           // persistent form: attempted steps between compactions
+    int solo;                 // persistent form: list entries [0, solo) run one per workgroup (workgroups [0, solo))
     CP dt0, dtmax, dtmin, grow, shrink;
     P doMinDt;
     CP rtol[Pos<SD>::value], atol[Pos<SD>::value];
@@ -95,20 +98,26 @@ struct Lane {
     int left;                             // time-sliced solve: attempted steps left in this launch
 
     // whole lane state to and from HBM (time-sliced solve), bit for bit
+    // one 16-byte aligned record per instance (rare: once per slice), moved
+    // with 16-byte accesses from one base address per lane
+    static constexpr int kSavedWords = (int)((sizeof(Lane) / 4 + 3) & ~(size_t)3);
     __device__ __forceinline__ void save(const A& a) const
     {
-        constexpr int NW = (int)(sizeof(Lane) / 4);
-        uint32_t w[NW];
+        uint32_t w[kSavedWords] = {};
         __builtin_memcpy(w, this, sizeof(Lane));
+        uint4* dst = reinterpret_cast<uint4*>(a.saved + (size_t)tid * kSavedWords);
 #pragma unroll
-        for (int i = 0; i < NW; i++) a.saved[(size_t)i * a.NT + tid] = w[i];
+        for (int i = 0; i < kSavedWords / 4; i++) dst[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
     }
     __device__ __forceinline__ void restore(const A& a, int id)
     {
-        constexpr int NW = (int)(sizeof(Lane) / 4);
-        uint32_t w[NW];
+        uint32_t w[kSavedWords];
+        const uint4* src = reinterpret_cast<const uint4*>(a.saved + (size_t)id * kSavedWords);
 #pragma unroll
-        for (int i = 0; i < NW; i++) w[i] = a.saved[(size_t)i * a.NT + id];
+        for (int i = 0; i < kSavedWords / 4; i++) {
+            const uint4 v = src[i];
+            w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
+        }
         __builtin_memcpy(this, w, sizeof(Lane));
     }
     // the instance's first launch initialises it, later launches resume it
@@ -539,7 +548,12 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
 
     L_t L;
     bool busy = false;
-    if (tid == 0) ctl[18] = 0u;
+    unsigned packed = ~0u;       // busy lanes right after the last exchange (they sit in lanes [0, packed))
+    const unsigned D = (unsigned)a.solo;
+    if (tid == 0) {
+        ctl[18] = 0u;
+        ctl[20] = blockIdx.x < D ? 1u : 0u;   // solo phase: lane 0 alone runs one of the first D entries
+    }
     for (;;) {
         // ---- 1. count live lanes
         const unsigned long long live_mask = __ballot(busy);
@@ -555,11 +569,18 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
         // ---- 2. reserve fresh instances for the idle lanes
         if (tid == 0) {
             unsigned base = 0, fresh = 0;
-            const unsigned need = (unsigned)B - live;
-            if (!ctl[18] && need > 0) {
-                base = atomicAdd(a.queue, need);
-                if (base < (unsigned)a.listLen) fresh = min(need, (unsigned)a.listLen - base);
-                if (base + need >= (unsigned)a.listLen) ctl[18] = 1u;
+            if (ctl[20] && live == 0) {
+                const unsigned q = atomicAdd(&a.queue[2], 1u);
+                if (q < D) { base = q; fresh = 1u; }
+                else ctl[20] = 0u;                  // solo entries taken: a full workgroup from now on
+            }
+            if (!ctl[20]) {
+                const unsigned need = (unsigned)B - live;
+                if (!ctl[18] && need > 0) {
+                    base = D + atomicAdd(a.queue, need);
+                    if (base < (unsigned)a.listLen) fresh = min(need, (unsigned)a.listLen - base);
+                    if (base + need >= (unsigned)a.listLen) ctl[18] = 1u;
+                }
             }
             ctl[16] = base;
             ctl[17] = fresh;
@@ -567,33 +588,44 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
         __syncthreads();
         const unsigned base = ctl[16], fresh = ctl[17];
         if (live == 0 && fresh == 0) break;           // block-uniform: queue drained, nothing live
-        // ---- 3. pack live states into lanes [0, live), fresh ones into [live, live + fresh)
-        uint32_t w[NCH * CW];
-        __builtin_memcpy(w, &L, sizeof(L_t));
+        // ---- 3. pack live states into lanes [0, live), fresh ones into [live, live + fresh);
+        // skipped (block-uniform) when no lane finished since the last exchange and
+        // nothing new arrives: the busy lanes are still packed
+        if (fresh != 0 || live != packed) {
+            // the lane state moves in place, CW words per pass (no second copy of it)
+            unsigned char* lw = reinterpret_cast<unsigned char*>(&L);
 #pragma unroll
-        for (int c = 0; c < NCH; c++) {
-            if (busy) {
+            for (int c = 0; c < NCH; c++) {
+                constexpr int last = NW - (NCH - 1) * CW;
+                const int nw = c == NCH - 1 ? last : CW;
+                uint32_t w[CW];
+                __builtin_memcpy(w, lw + c * CW * 4, nw * 4);
+                if (busy) {
 #pragma unroll
-                for (int j = 0; j < CW; j++) xch[j * B + rank] = w[c * CW + j];
+                    for (int j = 0; j < CW; j++)
+                        if (j < nw) xch[j * B + rank] = w[j];
+                }
+                __syncthreads();
+                if ((unsigned)tid < live) {
+#pragma unroll
+                    for (int j = 0; j < CW; j++)
+                        if (j < nw) w[j] = xch[j * B + tid];
+                    __builtin_memcpy(lw + c * CW * 4, w, nw * 4);
+                }
+                __syncthreads();
             }
-            __syncthreads();
             if ((unsigned)tid < live) {
-#pragma unroll
-                for (int j = 0; j < CW; j++) w[c * CW + j] = xch[j * B + tid];
+                busy = true;
+            } else if ((unsigned)tid < live + fresh) {
+                const int q = (int)(base + ((unsigned)tid - live));
+                const int id = a.list ? a.list[q] : q;
+                if constexpr (SLICED) L.begin(a, id, sPAR, sPARi);
+                else L.load(a, id, sPAR, sPARi);
+                busy = true;
+            } else {
+                busy = false;
             }
-            __syncthreads();
-        }
-        if ((unsigned)tid < live) {
-            __builtin_memcpy(&L, w, sizeof(L_t));
-            busy = true;
-        } else if ((unsigned)tid < live + fresh) {
-            const int q = (int)(base + ((unsigned)tid - live));
-            const int id = a.list ? a.list[q] : q;
-            if constexpr (SLICED) L.begin(a, id, sPAR, sPARi);
-            else L.load(a, id, sPAR, sPARi);
-            busy = true;
-        } else {
-            busy = false;
+            packed = live + fresh;
         }
         // ---- 4. integrate
         for (int m = 0; m < a.roundSteps; m++) {
@@ -623,7 +655,7 @@ constexpr size_t persistent_lds_bytes(int block)
 template <int Unused = 0>
 __global__ void reset_queue_kernel(unsigned int* q)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) q[0] = 0u;
+    if (threadIdx.x == 0 && blockIdx.x == 0) { q[0] = 0u; q[2] = 0u; }
 }
 
 } // namespace pt

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 #include <string>
@@ -139,6 +140,10 @@ public:
     std::vector<double> rtol, atol, etol;
     std::vector<int> edir;
     int kernelMode = MPGOS_KERNEL_PLAIN, wavesPerSimd = 0, graphCapture = 0, evcap = 0, variant = 0, roundSteps = 32;
+    int queueOrder = MPGOS_QUEUE_INDEX, soloInstances = 0;
+    int* order = nullptr;            // work-queue order of the persistent form (device), when not index order
+    std::vector<int> orderHost, rankHost;
+    bool orderDirty = true;
 
     Buffer<P> TD, X, T, cPAR, sPAR, ACC, DOt, DOx, evlog;
     Buffer<int> sPARi, ACCi, DOidx, status, nev, evlogidx;
@@ -168,7 +173,7 @@ public:
         if (ckpt) (void)hipFree(ckpt);
         if (seg) (void)hipFree(seg);
         if (queue) (void)hipFree(queue);
-        for (void* q : {(void*)saved, (void*)phase, (void*)listA, (void*)listB})
+        for (void* q : {(void*)saved, (void*)phase, (void*)listA, (void*)listB, (void*)order})
             if (q) (void)hipFree(q);
         if (syncEvent) (void)hipEventDestroy(syncEvent);
         if (evStart) (void)hipEventDestroy(evStart);
@@ -231,7 +236,7 @@ public:
         case MPGOS_PT_INITIAL_TIME_STEP: dt0 = v; return MPGOS_OK;
         case MPGOS_PT_ACTIVE_NUMBER_OF_THREADS:
             if ((int)v < 0 || (int)v > NT) { set_error("ActiveNumberOfThreads must be in 0..NT"); return MPGOS_E_ARG; }
-            nActive = (int)v; return MPGOS_OK;
+            nActive = (int)v; orderDirty = true; return MPGOS_OK;
         case MPGOS_PT_MAXIMUM_TIME_STEP: dtmax = v; return MPGOS_OK;
         case MPGOS_PT_MINIMUM_TIME_STEP: dtmin = v; return MPGOS_OK;
         case MPGOS_PT_TIME_STEP_GROW_LIMIT: grow = v; return MPGOS_OK;
@@ -261,6 +266,12 @@ public:
         case MPGOS_PT_PERSISTENT_ROUND_STEPS:
             if ((int)v < 1) { set_error("PersistentRoundSteps must be >= 1"); return MPGOS_E_ARG; }
             roundSteps = (int)v; return MPGOS_OK;
+        case MPGOS_PT_QUEUE_ORDER:
+            if ((int)v != MPGOS_QUEUE_INDEX && (int)v != MPGOS_QUEUE_LONGEST_FIRST) { set_error("QueueOrder must be 0 or 1"); return MPGOS_E_ARG; }
+            queueOrder = (int)v; orderDirty = true; return MPGOS_OK;
+        case MPGOS_PT_PERSISTENT_SOLO_INSTANCES:
+            if ((int)v < 0) { set_error("PersistentSoloInstances must be >= 0"); return MPGOS_E_ARG; }
+            soloInstances = (int)v; return MPGOS_OK;
         case MPGOS_PT_LAUNCH_VARIANT:
             if ((int)v < 0 || (int)v > 3) { set_error("LaunchVariant must be in 0..3"); return MPGOS_E_ARG; }
             variant = (int)v; return MPGOS_OK;
@@ -422,6 +433,7 @@ public:
     {
         MPGOS_HIP_TRY(hipSetDevice(device));
         int rc = MPGOS_OK;
+        if (to_device && (var == MPGOS_PT_TIME_DOMAIN || var == MPGOS_PT_ALL)) orderDirty = true;
         switch (var) {
         case MPGOS_PT_TIME_DOMAIN: return copy(TD, to_device);
         case MPGOS_PT_ACTUAL_STATE: return copy(X, to_device);
@@ -477,6 +489,7 @@ public:
         a.list = nullptr; a.listLen = nActive; a.slice = 0;
         a.doFreq = doFreq;
         a.roundSteps = roundSteps;
+        a.solo = 0;
         a.dt0 = (CP)(P)dt0;   // options are stored as Precision in the reference (PT-IF:619-650)
         a.dtmax = (CP)(P)dtmax; a.dtmin = (CP)(P)dtmin; a.grow = (CP)(P)grow; a.shrink = (CP)(P)shrink;
         if (KAHAN) { a.dt0 = (CP)dt0; a.dtmax = (CP)dtmax; a.dtmin = (CP)dtmin; a.grow = (CP)grow; a.shrink = (CP)shrink; }
@@ -557,15 +570,53 @@ public:
         return MPGOS_OK;
     }
 
+    // list entries run one per workgroup: at most one per workgroup of the grid
+    int solo_entries(int grid, int listLen) const
+    {
+        if (kernelMode != MPGOS_KERNEL_PERSISTENT) return 0;
+        int d = soloInstances < grid ? soloInstances : grid;
+        return d < listLen ? d : listLen;
+    }
+
+    // Work-queue order of the persistent form: the active instances sorted by
+    // decreasing integration interval (ties by index), from the time domains
+    // last uploaded.  Recomputed after the time domains, the active count or
+    // the option change; the order only schedules work.
+    bool ordered() const { return kernelMode == MPGOS_KERNEL_PERSISTENT && queueOrder == MPGOS_QUEUE_LONGEST_FIRST; }
+    int ensure_order()
+    {
+        if (!orderDirty && order) return MPGOS_OK;
+        if (!order) {
+            if (hipMalloc((void**)&order, sizeof(int) * (size_t)NT) != hipSuccess) { set_error("QueueOrder: allocation failed"); return MPGOS_E_MEMORY; }
+        }
+        orderHost.resize((size_t)nActive);
+        for (int i = 0; i < nActive; i++) orderHost[(size_t)i] = i;
+        const P* td = TD.h;
+        std::stable_sort(orderHost.begin(), orderHost.end(), [&](int x, int y) {
+            return (double)(td[x + NT] - td[x]) > (double)(td[y + NT] - td[y]);
+        });
+        rankHost.assign((size_t)NT, 0);
+        for (int r = 0; r < nActive; r++) rankHost[(size_t)orderHost[(size_t)r]] = r;
+        MPGOS_HIP_TRY(hipStreamSynchronize(strm));      // a running Solve may still read the old order
+        if (nActive > 0) MPGOS_HIP_TRY(hipMemcpy(order, orderHost.data(), sizeof(int) * (size_t)nActive, hipMemcpyHostToDevice));
+        orderDirty = false;
+        return MPGOS_OK;
+    }
+
     // PT-IF:1279-1285
     int solve() override
     {
         MPGOS_HIP_TRY(hipSetDevice(device));
         if (nActive == 0) return MPGOS_OK;
-        const Args<Sys, P, CP> a = make_args();
+        if (ordered()) {
+            if (int rc = ensure_order()) return rc;
+        }
+        Args<Sys, P, CP> a = make_args();
+        if (ordered()) a.list = order;
         KernelFn fn = kernel();
         int grid = 0;
         if (int rc = launch_geometry(fn, grid)) return rc;
+        a.solo = solo_entries(grid, a.listLen);
         MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
         if (!graphCapture) {
             if (int rc = enqueue(a, fn, grid)) return rc;
@@ -607,8 +658,8 @@ public:
 
     static constexpr size_t lane_words()
     {
-        const size_t a = sizeof(Lane<Sys, P, KAHAN, MPGOS_RK4>), b = sizeof(Lane<Sys, P, KAHAN, MPGOS_RKCK45>);
-        return (a > b ? a : b) / 4;
+        const size_t a = Lane<Sys, P, KAHAN, MPGOS_RK4>::kSavedWords, b = Lane<Sys, P, KAHAN, MPGOS_RKCK45>::kSavedWords;
+        return a > b ? a : b;
     }
 
     // One slice of a Solve: every unfinished instance attempts at most
@@ -631,6 +682,11 @@ public:
         if (!sliceActive) {
             MPGOS_HIP_TRY(hipMemsetAsync(phase, 0, 4 * (size_t)NT, strm));
             sliceList = nullptr;
+            if (ordered()) {
+                if (int rc = ensure_order()) return rc;
+                MPGOS_HIP_TRY(hipMemcpyAsync(listA, order, sizeof(int) * (size_t)nActive, hipMemcpyDeviceToDevice, strm));
+                sliceList = listA;
+            }
             sliceLen = nActive;
             sliceActive = true;
         }
@@ -642,7 +698,8 @@ public:
         int grid = 0;
         if (int rc = launch_geometry(fn, grid)) return rc;
         if (kernelMode == MPGOS_KERNEL_PLAIN) grid = (sliceLen + block() - 1) / block();
-        MPGOS_HIP_TRY(hipMemsetAsync(queue, 0, 8, strm));
+        a.solo = solo_entries(grid, sliceLen);
+        MPGOS_HIP_TRY(hipMemsetAsync(queue, 0, 12, strm));
         MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
         hipLaunchKernelGGL(fn, dim3(grid), dim3(block()), dyn_lds(), strm, a);
         MPGOS_HIP_TRY(hipGetLastError());
@@ -652,6 +709,13 @@ public:
         MPGOS_HIP_TRY(hipStreamSynchronize(strm));
         sliceList = a.listOut;
         sliceLen = (int)left;
+        if (ordered() && sliceLen > 1) {
+            // the kernel appends suspended instances in completion order: restore the queue order
+            std::vector<int> l((size_t)sliceLen);
+            MPGOS_HIP_TRY(hipMemcpy(l.data(), a.listOut, sizeof(int) * l.size(), hipMemcpyDeviceToHost));
+            std::sort(l.begin(), l.end(), [&](int x, int y) { return rankHost[(size_t)x] < rankHost[(size_t)y]; });
+            MPGOS_HIP_TRY(hipMemcpy(a.listOut, l.data(), sizeof(int) * l.size(), hipMemcpyHostToDevice));
+        }
         if (sliceLen == 0) sliceActive = false;
         *unfinished = sliceLen;
         return MPGOS_OK;

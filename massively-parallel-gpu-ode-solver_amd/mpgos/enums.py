@@ -62,6 +62,8 @@ class ListOfSolverOptions(enum.IntEnum):
     EventLogCapacity = 17
     LaunchVariant = 18
     PersistentRoundSteps = 19
+    QueueOrder = 20
+    PersistentSoloInstances = 21
 
 
 class KernelMode(enum.IntEnum):

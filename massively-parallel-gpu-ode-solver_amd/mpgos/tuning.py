@@ -90,6 +90,8 @@ def apply(solver, cfg):
     solver.SolverOption(O.LaunchVariant, cfg.get("variant", 0))
     solver.SolverOption(O.PersistentRoundSteps, cfg.get("round", 32))
     solver.SolverOption(O.PersistentWavesPerSimd, cfg.get("waves_per_simd", 0))
+    solver.SolverOption(O.QueueOrder, cfg.get("order", 0))
+    solver.SolverOption(O.PersistentSoloInstances, cfg.get("solo", 0))
 
 
 def time_config(solver, snap, cfg, repeats=2, warmup=1):

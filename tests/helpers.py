@@ -6,7 +6,7 @@ from mpgos import workloads as W
 
 
 def run_gpu(spec, precision=Precision.DOUBLE, kernel_mode=0, block=None, variant=0, graph=0, solves=1,
-            waves_per_simd=0, slice_steps=0):
+            waves_per_simd=0, slice_steps=0, queue_order=0, round_steps=0, solo=0):
     s = ProblemSolver(spec["system"], spec["nt"], spec["algorithm"], precision, ndo=spec.get("ndo", 0))
     try:
         W.load(s, spec)
@@ -14,6 +14,10 @@ def run_gpu(spec, precision=Precision.DOUBLE, kernel_mode=0, block=None, variant
         s.SolverOption(O.LaunchVariant, variant)
         s.SolverOption(O.GraphCapture, graph)
         s.SolverOption(O.PersistentWavesPerSimd, waves_per_simd)
+        s.SolverOption(O.QueueOrder, queue_order)
+        s.SolverOption(O.PersistentSoloInstances, solo)
+        if round_steps:
+            s.SolverOption(O.PersistentRoundSteps, round_steps)
         if block:
             s.SolverOption(O.ThreadsPerBlock, block)
         outs = []

@@ -63,6 +63,12 @@ def test_enumerations_follow_the_reference_order():
     assert [ListOfSolverOptions(i).name for i in range(len(ref_opts))] == ref_opts
     header = open(os.path.join(ROOT, "include", "mpgos.h")).read()
     assert re.search(r"MPGOS_PT_ALL = 0,\s*MPGOS_PT_TIME_DOMAIN,\s*MPGOS_PT_ACTUAL_STATE,", header)
+    # options past the reference's: the C enum order equals the Python enum's values
+    body = header[header.index("MPGOS_PT_THREADS_PER_BLOCK = 0"):header.index("MPGOS_PT_NUM_OPTIONS")]
+    names = re.findall(r"^\s*(MPGOS_PT_[A-Z_]+)", body, re.M)
+    assert names.index("MPGOS_PT_PERSISTENT_ROUND_STEPS") == ListOfSolverOptions.PersistentRoundSteps
+    assert names.index("MPGOS_PT_QUEUE_ORDER") == ListOfSolverOptions.QueueOrder
+    assert names.index("MPGOS_PT_PERSISTENT_SOLO_INSTANCES") == ListOfSolverOptions.PersistentSoloInstances
 
 
 def test_create_without_device_fails_cleanly():

@@ -103,6 +103,33 @@ def test_persistent_with_fewer_lanes_than_instances(gpu, oracle):
     assert_bitwise(out, ref, label="lorenz persistent refill")
 
 
+@pytest.mark.parametrize("solo", [0, 40], ids=["shared", "solo40"])
+@pytest.mark.parametrize("slice_steps", [0, 1500], ids=["whole", "sliced"])
+def test_persistent_longest_first_queue(gpu, oracle, slice_steps, solo):
+    """Work queue in longest-integration-interval-first order (uneven
+    horizons, one wave per SIMD so lanes refill, long rounds so compaction is
+    often skipped), optionally with the 40 longest instances alone in their
+    workgroups: results bit-identical to the oracle, whole and time-sliced."""
+    full = W.config2_duffing_uneven(nt=131072, evcap=256)
+    idx = np.argsort(full["TD"][1])[:1200]
+    spec = W.subset(full, idx[::-1].copy())             # longest horizons last in index order
+    print(f"longest-first queue slice_steps={slice_steps} solo={solo}", flush=True)
+    ref = oracle.run("duffing_eps", spec)
+    out = run_gpu(spec, kernel_mode=1, block=64, waves_per_simd=1, queue_order=1, round_steps=256,
+                  slice_steps=slice_steps, solo=solo)
+    assert_bitwise(out, ref, keys=("X", "T", "nacc", "nrej", "status", "nev", "evlog", "evlogidx"), label="longest first")
+
+
+def test_queue_order_option_is_checked(gpu):
+    from mpgos import ListOfSolverOptions as O, MPGOSError, ProblemSolver
+    s = ProblemSolver("lorenz", 64, 1)
+    try:
+        with pytest.raises(MPGOSError):
+            s.SolverOption(O.QueueOrder, 2)
+    finally:
+        s.close()
+
+
 def test_graph_replays_match_eager(gpu, oracle):
     from mpgos import ListOfSolverOptions as O, ListOfVariables as V, ProblemSolver
     spec = W.config1_lorenz(nt=2048)
