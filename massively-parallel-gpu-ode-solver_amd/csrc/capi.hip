@@ -19,6 +19,7 @@ static const Registration kSystems[] = {
     {"duffing_q", make_pt_duffing_q},
     {"duffing_eps", make_pt_duffing_eps},
     {"duffing_dw", make_pt_duffing_dw},
+    {"relief_valve", make_pt_relief_valve},
     {"lorenz", make_pt_lorenz},
     {"affine2", make_pt_affine2},
 };
@@ -43,6 +44,7 @@ __global__ void math_eval_kernel(int fn, const double* x, const double* y, doubl
     case 1: out[i] = math::sin(x[i]); break;
     case 2: out[i] = math::exp(x[i]); break;
     case 3: out[i] = math::log(x[i]); break;
+    case 5: out[i] = math::sqrt(x[i]); break;
     default: out[i] = math::pow(x[i], y[i]); break;
     }
 }
@@ -276,13 +278,14 @@ void* mpgos_pb_stream(mpgos_solver* h)
 
 int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, long n)
 {
-    if (fn < 0 || fn > 4 || !x || !out || (fn == 4 && !y)) { set_error("mpgos_math_eval_host: bad arguments"); return MPGOS_E_ARG; }
+    if (fn < 0 || fn > 5 || !x || !out || (fn == 4 && !y)) { set_error("mpgos_math_eval_host: bad arguments"); return MPGOS_E_ARG; }
     for (long i = 0; i < n; i++) {
         switch (fn) {
         case 0: out[i] = mpgos::math::cos(x[i]); break;
         case 1: out[i] = mpgos::math::sin(x[i]); break;
         case 2: out[i] = mpgos::math::exp(x[i]); break;
         case 3: out[i] = mpgos::math::log(x[i]); break;
+        case 5: out[i] = mpgos::math::sqrt(x[i]); break;
         default: out[i] = mpgos::math::pow(x[i], y[i]); break;
         }
     }
@@ -291,7 +294,7 @@ int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, 
 
 int mpgos_math_eval_device(int fn, const double* x_dev, const double* y_dev, double* out_dev, long n)
 {
-    if (fn < 0 || fn > 4 || !x_dev || !out_dev || (fn == 4 && !y_dev) || n < 0) {
+    if (fn < 0 || fn > 5 || !x_dev || !out_dev || (fn == 4 && !y_dev) || n < 0) {
         set_error("mpgos_math_eval_device: bad arguments");
         return MPGOS_E_ARG;
     }

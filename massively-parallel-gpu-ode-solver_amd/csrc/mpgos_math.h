@@ -54,6 +54,11 @@ MPGOS_HD MPGOS_INLINE bool isfinite(double x) { return (bits(x) & 0x7ff000000000
 MPGOS_HD MPGOS_INLINE bool isfinite(float x) { return (bits(x) & 0x7f800000u) != 0x7f800000u; }
 MPGOS_HD MPGOS_INLINE bool isnan(double x) { return x != x; }
 
+// IEEE square root: correctly rounded on the host and on gfx950 (the device
+// expansion of the square root refines the hardware estimate to the correctly
+// rounded result; tests compare both bit for bit).
+MPGOS_HD MPGOS_INLINE double sqrt(double x) { return __builtin_sqrt(x); }
+
 // 2^n for integer n in [-1022, 1023] built from the exponent field.
 MPGOS_HD MPGOS_INLINE double pow2i(int n) { return from_bits((uint64_t)(n + 1023) << 52); }
 
@@ -269,6 +274,7 @@ MPGOS_HD MPGOS_INLINE float sin(float x) { return (float)sin((double)x); }
 MPGOS_HD MPGOS_INLINE float exp(float x) { return (float)exp((double)x); }
 MPGOS_HD MPGOS_INLINE float log(float x) { return (float)log((double)x); }
 MPGOS_HD MPGOS_INLINE float pow(float x, float y) { return (float)pow((double)x, (double)y); }
+MPGOS_HD MPGOS_INLINE float sqrt(float x) { return (float)sqrt((double)x); }   // exact: double has > 2 * 24 + 2 bits
 
 } // namespace math
 } // namespace mpgos

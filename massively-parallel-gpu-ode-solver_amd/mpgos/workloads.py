@@ -24,6 +24,7 @@ SYSTEM_DIMS = {
     "duffing_q":   (3, 2, 0, 0, 0, 0, 0),
     "duffing_eps": (2, 3, 0, 0, 1, 0, 0),
     "duffing_dw":  (2, 1, 1, 0, 2, 3, 1),
+    "relief_valve": (3, 1, 4, 0, 2, 3, 2),
     "lorenz":      (3, 1, 0, 0, 0, 0, 0),
     "affine2":     (2, 6, 1, 0, 1, 0, 1),
 }
@@ -83,6 +84,25 @@ def new_spec(system, algorithm, nt, ndo=0):
         "sPAR": np.zeros(nsp), "sPARi": np.zeros(nisp, np.int32), "ACC": np.zeros((na, nt)),
         "ACCi": np.zeros((nia, nt), np.int32), "DOidx": np.zeros(nt, np.int32),
     }
+
+
+def tutorial_relief_valve(nt=256):
+    """The reference's impact-dynamics tutorial setup
+    (Tutorials_SingleSystem_PerThread/T6_ImpactDynamics/ImpactDynamics.cu:56-66,
+    166-186): pressure relief valve, flow rate q = linspace(0.2, 10) per
+    instance, (x, v, p) = (0.2, 0, 0), (kappa, delta, beta, r) =
+    (1.25, 10, 20, 0.8), time domain [0, 1e10] (a Solve ends at the next
+    maximum of x or when the valve rests), RKCK45 at rtol = atol = 1e-10, both
+    event directions -1; repeated Solves continue from the last state."""
+    s = new_spec("relief_valve", Algorithms.RKCK45, nt)
+    s["TD"][1] = 1.0e10
+    s["X"][0] = 0.2
+    s["cPAR"][0] = linspace(0.2, 10.0, nt)
+    s["sPAR"][:] = (1.25, 10.0, 20.0, 0.8)
+    s["rtol"] = 1e-10
+    s["atol"] = 1e-10
+    s["edir"] = np.array([-1, -1])
+    return s
 
 
 def subset(spec, idx):

@@ -66,6 +66,13 @@ template <int N> D<N> sin(const D<N>& a)
     for (int i = 0; i < N; i++) r.d[i] = c * a.d[i];
     return r;
 }
+template <int N> D<N> sqrt(const D<N>& a)
+{
+    D<N> r(mpgos::math::sqrt(a.v));
+    const double g = 0.5 / r.v;
+    for (int i = 0; i < N; i++) r.d[i] = g * a.d[i];
+    return r;
+}
 // only the controller calls pow; its result is frozen, so the value suffices
 template <int N> D<N> pow(const D<N>& a, const D<N>& b) { return D<N>(mpgos::math::pow(a.v, b.v)); }
 template <int N> D<N> frozen(const D<N>& a) { return D<N>(a.v); }

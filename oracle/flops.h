@@ -6,7 +6,7 @@
 // elementary functions are charged the operations their source performs
 // (csrc/mpgos_math.h): cos or sin 38 (reduction 5, both polynomial kernels
 // 19 + 14), exp 31, log 28, pow 70 (double-double log 32, product 5,
-// exp 31, final fma 2).  Used for the
+// exp 31, final fma 2), sqrt 1.  Used for the
 // roofline's algorithmic FLOPs per step.
 #pragma once
 
@@ -48,6 +48,7 @@ F sin(F a);
 F exp(F a);
 F log(F a);
 F pow(F a, F b);
+F sqrt(F a);
 
 } // namespace flops
 } // namespace oracle

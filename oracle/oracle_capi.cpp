@@ -15,6 +15,7 @@
 #include "systems/affine2.h"
 #include "systems/duffing.h"
 #include "systems/duffing_dw.h"
+#include "systems/relief_valve.h"
 #include "systems/lorenz.h"
 #include "systems/pb_ring.h"
 
@@ -26,6 +27,7 @@ F sin(F a) { g_count += 38; return F(mpgos::math::sin(a.v)); }
 F exp(F a) { g_count += 31; return F(mpgos::math::exp(a.v)); }
 F log(F a) { g_count += 28; return F(mpgos::math::log(a.v)); }
 F pow(F a, F b) { g_count += 70; return F(mpgos::math::pow(a.v, b.v)); }
+F sqrt(F a) { g_count += 1; return F(mpgos::math::sqrt(a.v)); }
 } // namespace flops
 
 template <class Sys, class S>
@@ -101,6 +103,7 @@ static const Entry kSystems[] = {
     {"duffing_q", run_all<mpgos::systems::DuffingQ, double>, run_all<mpgos::systems::DuffingQ, flops::F>, trace_entry<mpgos::systems::DuffingQ>, sensitivity_all<mpgos::systems::DuffingQ>},
     {"duffing_eps", run_all<mpgos::systems::DuffingEps, double>, run_all<mpgos::systems::DuffingEps, flops::F>, trace_entry<mpgos::systems::DuffingEps>, sensitivity_all<mpgos::systems::DuffingEps>},
     {"duffing_dw", run_all<mpgos::systems::DuffingDW, double>, run_all<mpgos::systems::DuffingDW, flops::F>, trace_entry<mpgos::systems::DuffingDW>, sensitivity_all<mpgos::systems::DuffingDW>},
+    {"relief_valve", run_all<mpgos::systems::ReliefValve, double>, run_all<mpgos::systems::ReliefValve, flops::F>, trace_entry<mpgos::systems::ReliefValve>, sensitivity_all<mpgos::systems::ReliefValve>},
     {"lorenz", run_all<mpgos::systems::Lorenz, double>, run_all<mpgos::systems::Lorenz, flops::F>, trace_entry<mpgos::systems::Lorenz>, sensitivity_all<mpgos::systems::Lorenz>},
     {"affine2", run_all<mpgos::systems::Affine2, double>, run_all<mpgos::systems::Affine2, flops::F>, trace_entry<mpgos::systems::Affine2>, sensitivity_all<mpgos::systems::Affine2>},
 };

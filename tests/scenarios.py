@@ -92,6 +92,8 @@ def all_scenarios():
     dw["edir"] = np.array([-1, 0])
     dw["options"].update(dtmax=1e3, dtmin=1e-14, grow=10.0, shrink=0.2)
     out.append(("tutorial_duffing_doublewell", "duffing_dw", dw))
+    # reference impact tutorial: state jump at impact, user termination (first maximum, or resting valve)
+    out.append(("tutorial_relief_valve", "relief_valve", W.tutorial_relief_valve(nt=256)))
     # ragged: only part of the instances active
     lz = W.config1_lorenz(nt=1000)
     lz["n_active"] = 777

@@ -60,7 +60,8 @@ def test_time_sliced_solve_is_bitwise(gpu, oracle, mode):
     assert_bitwise(out, ref, keys=("X", "T", "nacc", "nrej", "status", "nev", "evlog", "evlogidx"), label="sliced")
 
 
-@pytest.mark.parametrize("name", ["dense_output_f3_m0.0", "tutorial_duffing_doublewell", "event_ramp_both"])
+@pytest.mark.parametrize("name", ["dense_output_f3_m0.0", "tutorial_duffing_doublewell", "event_ramp_both",
+                                  "tutorial_relief_valve"])
 def test_time_sliced_scenarios(gpu, oracle, name):
     _, system, spec = next(x for x in S.all_scenarios() if x[0] == name)
     ref = oracle.run(system, spec)
@@ -162,6 +163,22 @@ def test_repeated_solves_continue_the_integration(gpu, oracle):
         cur["X"], cur["T"], cur["ACC"] = ref["X"], ref["T"], ref["ACC"]
 
 
+def test_relief_valve_continuation_solves(gpu, oracle):
+    # the impact tutorial chains Solve() calls, each from the last state to the
+    # next maximum (ImpactDynamics.cu:90-117); state jumps and user terminations
+    spec = W.tutorial_relief_valve(nt=256)
+    for mode in (0, 1):
+        print(f"relief valve continuation mode={mode}", flush=True)
+        outs = run_gpu(spec, kernel_mode=mode, solves=4)
+        cur = dict(spec)
+        for k, out in enumerate(outs):
+            ref = oracle.run("relief_valve", cur)
+            assert_bitwise(out, ref, keys=("X", "T", "ACC", "ACCi", "nacc", "nrej", "status"), label=f"solve {k} mode {mode}")
+            cur = dict(cur)
+            cur["X"], cur["T"] = ref["X"], ref["T"]
+        assert (outs[-1]["ACCi"][0] == 1).sum() > 200         # most instances ended at a maximum
+
+
 def test_device_math_matches_host_bitwise(gpu):
     import ctypes
     from test_math import FN, host_eval
@@ -177,7 +194,7 @@ def test_device_math_matches_host_bitwise(gpu):
     try:
         L = gpu.lib()
         for name, fn in FN.items():
-            arg = np.ascontiguousarray(np.abs(x) if name in ("log", "pow") else (np.clip(x, -700, 700) if name == "exp" else x))
+            arg = np.ascontiguousarray(np.abs(x) if name in ("log", "pow", "sqrt") else (np.clip(x, -700, 700) if name == "exp" else x))
             assert hip.hipMemcpy(bufs[0], arg.ctypes.data, nbytes, 1) == 0
             assert hip.hipMemcpy(bufs[1], y.ctypes.data, nbytes, 1) == 0
             assert L.mpgos_math_eval_device(fn, bufs[0], bufs[1], bufs[2], arg.size) == 0

@@ -13,7 +13,7 @@ import pytest
 
 from mpgos import lib
 
-FN = {"cos": 0, "sin": 1, "exp": 2, "log": 3, "pow": 4}
+FN = {"cos": 0, "sin": 1, "exp": 2, "log": 3, "pow": 4, "sqrt": 5}
 
 
 def host_eval(name, x, y=None):
@@ -108,3 +108,12 @@ def test_float_versions_round_once():
     x = rng.uniform(-100, 100, 1000).astype(np.float32).astype(np.float64)
     assert np.array_equal(host_eval("cos", x).astype(np.float32), np.cos(x).astype(np.float32)) or \
         np.abs(host_eval("cos", x).astype(np.float32) - np.cos(x).astype(np.float32)).max() <= 6e-8
+
+
+def test_sqrt_is_correctly_rounded():
+    rng = np.random.default_rng(11)
+    x = np.concatenate([10.0 ** rng.uniform(-308, 308, 20000), rng.uniform(0, 10, 20000),
+                        np.array([0.0, -0.0, np.inf, 5e-324, 2.2250738585072014e-308, 1.0, 4.0])])
+    out = host_eval("sqrt", x)
+    assert np.array_equal(out.view(np.uint64), np.sqrt(x).view(np.uint64))
+    assert np.isnan(host_eval("sqrt", np.array([-1.0, np.nan]))).all()

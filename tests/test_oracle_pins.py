@@ -287,3 +287,51 @@ def test_sensitivity_of_rk4_on_a_linear_system_is_the_matrix_power(oracle):
                       td=(0.0, 2.0), dt0=0.125, algorithm=Algorithms.RK4)
         xs.append(oracle.run("affine2", sp)["X"][:, 0])
     assert np.allclose((xs[0] - xs[1]) / (2 * eps), J[:, 2], rtol=1e-8, atol=1e-10)
+
+
+def _relief_valve_scipy(q, kappa=1.25, delta=10.0, beta=20.0, r=0.8, x0=(0.2, 0.0, 0.0)):
+    """The impact tutorial's first Solve with SciPy: integrate to the first
+    downward zero of v (terminal), restarting after every downward zero of x
+    with v -> -r v.  Returns (t_end, state, impacts)."""
+    def f(t, y):
+        return [y[1], -kappa * y[1] - (y[0] + delta) + y[2], beta * (q - y[0] * math.sqrt(max(y[2], 0.0)))]
+
+    def impact(t, y):
+        return y[0]
+    impact.terminal, impact.direction = True, -1
+
+    def peak(t, y):
+        return y[1]
+    peak.terminal, peak.direction = True, -1
+    # v starts at 0 and falls: a crossing needs a positive value before it (as in
+    # the solver's detection), so step off the initial zero first
+    first = solve_ivp(f, (0.0, 1e-6), np.array(x0, float), method="DOP853", rtol=1e-13, atol=1e-13)
+    t, y, impacts = 1e-6, first.y[:, -1], 0
+    for _ in range(100):
+        sol = solve_ivp(f, (t, t + 100.0), y, method="DOP853", rtol=1e-13, atol=1e-13, events=(impact, peak))
+        if sol.t_events[1].size:
+            return sol.t_events[1][0], sol.y_events[1][0], impacts
+        t, y = sol.t_events[0][0], sol.y_events[0][0].copy()
+        y[0] = 0.0
+        y[1] = -r * y[1]
+        impacts += 1
+    raise AssertionError("no peak")
+
+
+def test_relief_valve_impacts_and_user_termination_match_scipy(oracle):
+    """Impact tutorial (state jump at x = 0, user termination at the first
+    maximum of x): the oracle's end time and state agree with an event-driven
+    SciPy integration to the event tolerance's effect (1e-5); instances with
+    bounces before the maximum are among them."""
+    spec = W.tutorial_relief_valve(nt=12)
+    out = oracle.run("relief_valve", spec)
+    q = spec["cPAR"][0]
+    bounced = 0
+    for i in range(spec["nt"]):
+        t_end, y_end, impacts = _relief_valve_scipy(q[i])
+        bounced += impacts > 0
+        assert out["ACCi"][0, i] == 1 and out["ACCi"][1, i] < 50      # ended by the first maximum
+        assert abs(out["T"][i] - t_end) < 1e-5 * max(1.0, t_end), (i, out["T"][i], t_end)
+        assert np.allclose(out["X"][[0, 2], i], y_end[[0, 2]], rtol=1e-5, atol=1e-6), (i, out["X"][:, i], y_end)
+        assert abs(out["X"][1, i]) < 1e-6                              # v within the event tolerance
+    assert bounced >= 2
