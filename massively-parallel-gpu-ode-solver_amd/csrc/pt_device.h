@@ -31,6 +31,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include <type_traits>
 
 #include "mpgos_math.h"
@@ -64,6 +65,8 @@ struct Args {
     // (count in queue[1]); phase[id] = 2 once finished
     const int* list; int* listOut; uint32_t* saved; int* phase;
     int listLen, slice;
+    const unsigned* listLenDev;   // relaunch form: the list length is the previous launch's count (device)
+    unsigned* outCount;           // relaunch form: count of this launch's suspended instances (else queue[1])
     int NT, nActive, NDO, evcap, preferShared;
     int doFreq;
     int roundSteps;           // persistent form: attempted steps between compactions
@@ -79,6 +82,8 @@ template <class Sys, class P, bool KAHAN, int ALG>
 struct Lane {
     using CP = typename std::conditional<KAHAN, double, P>::type;
     using A = Args<Sys, P, CP>;
+    using Sys_t = Sys;
+    static constexpr bool kKahan = KAHAN;
     static constexpr int SD = Sys::SD, NCP = Sys::NCP, NE = Sys::NE, NA = Sys::NA, NIA = Sys::NIA;
 
     P TD[2];
@@ -138,7 +143,7 @@ struct Lane {
         if (--left <= 0) {
             save(a);
             a.phase[tid] = 1;
-            a.listOut[atomicAdd(&a.queue[1], 1u)] = tid;
+            a.listOut[atomicAdd(a.outCount ? a.outCount : &a.queue[1], 1u)] = tid;
             return true;
         }
         return false;
@@ -470,6 +475,24 @@ struct Lane {
     }
 };
 
+// Words of the lane state that an instantiation never uses (Kahan terms
+// without Kahan, accessories, event values and the slice budget of systems or
+// forms that have none): the persistent form does not move them between
+// lanes, so they hold no registers.
+template <class L_t, bool SLICED>
+__host__ __device__ constexpr bool lane_word_live(int w)
+{
+    using Sys = typename L_t::Sys_t;
+    const int b = 4 * w;
+    auto in = [&](size_t off, size_t bytes) { return (size_t)b >= off && (size_t)b < off + bytes; };
+    if (!L_t::kKahan && (in(offsetof(L_t, Xc), sizeof(L_t::Xc)) || in(offsetof(L_t, tc), sizeof(L_t::tc)))) return false;
+    if (Sys::NA == 0 && in(offsetof(L_t, ACC), sizeof(L_t::ACC))) return false;
+    if (Sys::NIA == 0 && in(offsetof(L_t, ACCi), sizeof(L_t::ACCi))) return false;
+    if (Sys::NE == 0 && in(offsetof(L_t, EF), sizeof(L_t::EF))) return false;
+    if (!SLICED && in(offsetof(L_t, left), sizeof(L_t::left))) return false;
+    return true;
+}
+
 // Stage shared parameters into LDS (PT-SOL:67-88) or point at global memory.
 template <class Sys, class P, class CP>
 __device__ __forceinline__ void stage_shared(const Args<Sys, P, CP>& a, P*& sPAR, int*& sPARi)
@@ -497,7 +520,8 @@ __device__ __forceinline__ void plain_body(const Args<Sys, P, typename Lane<Sys,
     P* sPAR; int* sPARi;
     stage_shared(a, sPAR, sPARi);
     const int i = threadIdx.x + blockIdx.x * blockDim.x;
-    if (i < a.listLen) {
+    const int n = (SLICED && a.listLenDev) ? (int)*a.listLenDev : a.listLen;
+    if (i < n) {
         Lane<Sys, P, KAHAN, ALG> L;
         const int id = a.list ? a.list[i] : i;
         if constexpr (SLICED) {
@@ -518,10 +542,12 @@ __device__ __forceinline__ void plain_body(const Args<Sys, P, typename Lane<Sys,
 // work queue; every `roundSteps` attempted steps the workgroup
 //   1. counts its live lanes (wavefront ballots, prefix over waves in LDS),
 //   2. reserves as many fresh instances as it has idle lanes (one atomic per
-//      workgroup),
-//   3. packs the live instances into the lowest lanes through LDS, followed by
-//      the fresh ones, so the waves above the packed range have nothing to do
-//      and sleep at the barrier instead of issuing instructions for idle lanes.
+//      workgroup) and starts them in the idle lanes where they are (no state
+//      moves while the queue lasts),
+//   3. once nothing new arrives, packs the live instances into the lowest
+//      lanes through LDS -- only when that empties at least one wave, so the
+//      waves above the packed range have nothing to do and sleep at the
+//      barrier instead of issuing instructions for idle lanes.
 // Every instance's arithmetic is independent of the lane that runs it, so the
 // results are bit-identical to the plain form.  The workgroup leaves the loop
 // together once the queue is drained and no lane is live (block-uniform test).
@@ -548,7 +574,6 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
 
     L_t L;
     bool busy = false;
-    unsigned packed = ~0u;       // busy lanes right after the last exchange (they sit in lanes [0, packed))
     const unsigned D = (unsigned)a.solo;
     if (tid == 0) {
         ctl[18] = 0u;
@@ -559,13 +584,16 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
         const unsigned long long live_mask = __ballot(busy);
         if (lane == 0) ctl[wave] = (unsigned)__popcll(live_mask);
         __syncthreads();
-        unsigned before = 0, live = 0;
+        unsigned before = 0, live = 0, occupied = 0;
         for (int w = 0; w < nwaves; w++) {
             const unsigned c = ctl[w];
             if (w < wave) before += c;
             live += c;
+            occupied += c > 0 ? 1u : 0u;
         }
-        const unsigned rank = before + (unsigned)__popcll(live_mask & ((1ull << lane) - 1ull));
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const unsigned rank = before + (unsigned)__popcll(live_mask & below);             // among live lanes
+        const unsigned idleRank = (unsigned)(wave * 64) - before + (unsigned)__popcll(~live_mask & below);   // among idle lanes
         // ---- 2. reserve fresh instances for the idle lanes
         if (tid == 0) {
             unsigned base = 0, fresh = 0;
@@ -588,44 +616,45 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
         __syncthreads();
         const unsigned base = ctl[16], fresh = ctl[17];
         if (live == 0 && fresh == 0) break;           // block-uniform: queue drained, nothing live
-        // ---- 3. pack live states into lanes [0, live), fresh ones into [live, live + fresh);
-        // skipped (block-uniform) when no lane finished since the last exchange and
-        // nothing new arrives: the busy lanes are still packed
-        if (fresh != 0 || live != packed) {
+        // ---- 3. fresh instances start in the idle lanes where they are; with
+        // nothing new, live states are packed into lanes [0, live) when that
+        // empties a wave (block-uniform decision)
+        if (fresh != 0) {
+            if (!busy && idleRank < fresh) {
+                const int q = (int)(base + idleRank);
+                const int id = a.list ? a.list[q] : q;
+                if constexpr (SLICED) L.begin(a, id, sPAR, sPARi);
+                else L.load(a, id, sPAR, sPARi);
+                busy = true;
+            }
+        } else if (occupied > (live + 63u) / 64u) {
             // the lane state moves in place, CW words per pass (no second copy of it)
             unsigned char* lw = reinterpret_cast<unsigned char*>(&L);
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
                 constexpr int last = NW - (NCH - 1) * CW;
                 const int nw = c == NCH - 1 ? last : CW;
+                bool any = false;
+#pragma unroll
+                for (int j = 0; j < CW; j++) any = any || (j < nw && lane_word_live<L_t, SLICED>(c * CW + j));
+                if (!any) continue;                   // compile-time: a chunk of unused words
                 uint32_t w[CW];
                 __builtin_memcpy(w, lw + c * CW * 4, nw * 4);
                 if (busy) {
 #pragma unroll
                     for (int j = 0; j < CW; j++)
-                        if (j < nw) xch[j * B + rank] = w[j];
+                        if (j < nw && lane_word_live<L_t, SLICED>(c * CW + j)) xch[j * B + rank] = w[j];
                 }
                 __syncthreads();
                 if ((unsigned)tid < live) {
 #pragma unroll
                     for (int j = 0; j < CW; j++)
-                        if (j < nw) w[j] = xch[j * B + tid];
+                        if (j < nw && lane_word_live<L_t, SLICED>(c * CW + j)) w[j] = xch[j * B + tid];
                     __builtin_memcpy(lw + c * CW * 4, w, nw * 4);
                 }
                 __syncthreads();
             }
-            if ((unsigned)tid < live) {
-                busy = true;
-            } else if ((unsigned)tid < live + fresh) {
-                const int q = (int)(base + ((unsigned)tid - live));
-                const int id = a.list ? a.list[q] : q;
-                if constexpr (SLICED) L.begin(a, id, sPAR, sPARi);
-                else L.load(a, id, sPAR, sPARi);
-                busy = true;
-            } else {
-                busy = false;
-            }
-            packed = live + fresh;
+            busy = (unsigned)tid < live;
         }
         // ---- 4. integrate
         for (int m = 0; m < a.roundSteps; m++) {
@@ -648,6 +677,16 @@ template <class Sys, class P, bool KAHAN, int ALG>
 constexpr size_t persistent_lds_bytes(int block)
 {
     return 32 * 4 + (size_t)4 * 4 * block;
+}
+
+// Clears the per-instance phases and the per-launch counts of the relaunch
+// form (a kernel, so the launch sequence can be captured in a graph).
+template <int Unused = 0>
+__global__ void reset_relaunch_kernel(int* phase, int n, unsigned* counts, int nc)
+{
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i < n) phase[i] = 0;
+    if (i < nc) counts[i] = 0u;
 }
 
 // Resets the work-queue head on the stream (a kernel, so the sequence can be

@@ -134,13 +134,16 @@ public:
     int blockSize = 64, nActive = 0, preferShared = 1;
     bool blockSet = false;                 // ThreadsPerBlock given explicitly
     static constexpr int kPersistentBlock = 256;
-    int block() const { return (kernelMode == MPGOS_KERNEL_PLAIN || blockSet) ? blockSize : kPersistentBlock; }
+    int block() const { return (kernelMode != MPGOS_KERNEL_PERSISTENT || blockSet) ? blockSize : kPersistentBlock; }
     double dt0 = 1e-2, dtmax = 1.0e6, dtmin = 1.0e-12, grow = 5.0, shrink = 0.1, doMinDt = 0.0;
     int doFreq = 1;
     std::vector<double> rtol, atol, etol;
     std::vector<int> edir;
     int kernelMode = MPGOS_KERNEL_PLAIN, wavesPerSimd = 0, graphCapture = 0, evcap = 0, variant = 0, roundSteps = 32;
     int queueOrder = MPGOS_QUEUE_INDEX, soloInstances = 0;
+    int relaunchSteps = 300, relaunchLaunches = 16;
+    unsigned* relaunchCounts = nullptr;   // per-launch suspended counts of the relaunch form
+    int relaunchCountsN = 0;
     int* order = nullptr;            // work-queue order of the persistent form (device), when not index order
     std::vector<int> orderHost, rankHost;
     bool orderDirty = true;
@@ -159,7 +162,7 @@ public:
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     std::vector<unsigned char> graphArgs;
-    int graphKey = -1;
+    long long graphKey = -1;
 
     ~Solver() override
     {
@@ -173,7 +176,7 @@ public:
         if (ckpt) (void)hipFree(ckpt);
         if (seg) (void)hipFree(seg);
         if (queue) (void)hipFree(queue);
-        for (void* q : {(void*)saved, (void*)phase, (void*)listA, (void*)listB, (void*)order})
+        for (void* q : {(void*)saved, (void*)phase, (void*)listA, (void*)listB, (void*)order, (void*)relaunchCounts})
             if (q) (void)hipFree(q);
         if (syncEvent) (void)hipEventDestroy(syncEvent);
         if (evStart) (void)hipEventDestroy(evStart);
@@ -257,7 +260,10 @@ public:
         case MPGOS_PT_DENSE_OUTPUT_SAVE_FREQUENCY: doFreq = (int)v; return MPGOS_OK;
         case MPGOS_PT_PREFER_SHARED_MEMORY: preferShared = (int)v != 0; return MPGOS_OK;
         case MPGOS_PT_KERNEL_MODE:
-            if ((int)v != MPGOS_KERNEL_PLAIN && (int)v != MPGOS_KERNEL_PERSISTENT) { set_error("KernelMode must be 0 or 1"); return MPGOS_E_ARG; }
+            if ((int)v != MPGOS_KERNEL_PLAIN && (int)v != MPGOS_KERNEL_PERSISTENT && (int)v != MPGOS_KERNEL_RELAUNCH) {
+                set_error("KernelMode must be 0, 1 or 2");
+                return MPGOS_E_ARG;
+            }
             kernelMode = (int)v; return MPGOS_OK;
         case MPGOS_PT_PERSISTENT_WAVES_PER_SIMD:
             if ((int)v < 0 || (int)v > 8) { set_error("PersistentWavesPerSimd must be in 0..8"); return MPGOS_E_ARG; }
@@ -269,6 +275,12 @@ public:
         case MPGOS_PT_QUEUE_ORDER:
             if ((int)v != MPGOS_QUEUE_INDEX && (int)v != MPGOS_QUEUE_LONGEST_FIRST) { set_error("QueueOrder must be 0 or 1"); return MPGOS_E_ARG; }
             queueOrder = (int)v; orderDirty = true; return MPGOS_OK;
+        case MPGOS_PT_RELAUNCH_STEPS:
+            if ((int)v < 1) { set_error("RelaunchSteps must be >= 1"); return MPGOS_E_ARG; }
+            relaunchSteps = (int)v; return MPGOS_OK;
+        case MPGOS_PT_RELAUNCH_LAUNCHES:
+            if ((int)v < 0 || (int)v > 4096) { set_error("RelaunchLaunches must be in 0..4096"); return MPGOS_E_ARG; }
+            relaunchLaunches = (int)v; return MPGOS_OK;
         case MPGOS_PT_PERSISTENT_SOLO_INSTANCES:
             if ((int)v < 0) { set_error("PersistentSoloInstances must be >= 0"); return MPGOS_E_ARG; }
             soloInstances = (int)v; return MPGOS_OK;
@@ -486,7 +498,7 @@ public:
         a.nacc = nacc.d; a.nrej = nrej.d; a.status = status.d; a.nev = nev.d;
         a.evlog = evlog.d; a.evlogidx = evlogidx.d; a.queue = queue;
         a.NT = NT; a.nActive = nActive; a.NDO = NDO; a.evcap = evcap; a.preferShared = preferShared;
-        a.list = nullptr; a.listLen = nActive; a.slice = 0;
+        a.list = nullptr; a.listLen = nActive; a.slice = 0; a.listLenDev = nullptr; a.outCount = nullptr;
         a.doFreq = doFreq;
         a.roundSteps = roundSteps;
         a.solo = 0;
@@ -517,7 +529,8 @@ public:
 
     KernelFn kernel(bool sliced = false) const
     {
-        const bool plain = kernelMode == MPGOS_KERNEL_PLAIN;
+        const bool plain = kernelMode != MPGOS_KERNEL_PERSISTENT;
+        sliced = sliced || kernelMode == MPGOS_KERNEL_RELAUNCH;     // the relaunch form runs the sliced lanes
         if (algorithm == MPGOS_RK4) {
             if (sliced) return plain ? pick_variant<MPGOS_RK4, MPGOS_KERNEL_PLAIN + 2>(variant)
                                      : pick_variant<MPGOS_RK4, MPGOS_KERNEL_PERSISTENT + 2>(variant);
@@ -533,7 +546,7 @@ public:
     size_t dyn_lds() const
     {
         const size_t sharedBytes = preferShared ? (size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int) : 0;
-        if (kernelMode == MPGOS_KERNEL_PLAIN) return sharedBytes;
+        if (kernelMode != MPGOS_KERNEL_PERSISTENT) return sharedBytes;
         // persistent form: shared parameters (16-byte aligned), control words, exchange buffer
         return ((((size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int)) + 15) & ~(size_t)15) +
                persistent_lds_bytes<Sys, P, KAHAN, MPGOS_RK4>(block());
@@ -544,6 +557,10 @@ public:
         const int B = block();
         if (kernelMode == MPGOS_KERNEL_PLAIN) {
             grid = NT / B + (NT % B == 0 ? 0 : 1);   // PT-IF:344
+            return MPGOS_OK;
+        }
+        if (kernelMode == MPGOS_KERNEL_RELAUNCH) {   // lists are at most nActive long
+            grid = nActive / B + (nActive % B == 0 ? 0 : 1);
             return MPGOS_OK;
         }
         int perCU = 0;
@@ -561,8 +578,55 @@ public:
         return MPGOS_OK;
     }
 
+    // Relaunch form: relaunchLaunches launches of relaunchSteps attempted steps
+    // per instance, each working on the previous launch's suspended instances
+    // packed into the lowest lanes (list length read on the device), then one
+    // launch without a budget; queued together, no host round trip.
+    int enqueue_relaunch(const Args<Sys, P, CP>& a, KernelFn fn, int grid)
+    {
+        const int S = relaunchLaunches;
+        hipLaunchKernelGGL(reset_relaunch_kernel<0>, dim3((unsigned)((NT > S + 1 ? NT : S + 1) + 255) / 256), dim3(256), 0, strm,
+                           phase, NT, relaunchCounts, S + 1);
+        for (int s = 0; s <= S; s++) {
+            Args<Sys, P, CP> b = a;
+            b.saved = saved; b.phase = phase;
+            b.list = s == 0 ? a.list : (s % 2 == 1 ? listA : listB);
+            b.listOut = s % 2 == 0 ? listA : listB;
+            b.listLen = nActive;
+            b.listLenDev = s == 0 ? nullptr : relaunchCounts + (s - 1);
+            b.outCount = relaunchCounts + s;
+            b.slice = s < S ? relaunchSteps : 0x7fffffff;
+            hipLaunchKernelGGL(fn, dim3(grid), dim3(block()), dyn_lds(), strm, b);
+        }
+        MPGOS_HIP_TRY(hipGetLastError());
+        return MPGOS_OK;
+    }
+
+    int ensure_slice_buffers()
+    {
+        if (!saved) {
+            if (hipMalloc((void**)&saved, lane_words() * 4 * (size_t)NT) != hipSuccess || hipMalloc((void**)&phase, 4 * (size_t)NT) != hipSuccess ||
+                hipMalloc((void**)&listA, 4 * (size_t)NT) != hipSuccess || hipMalloc((void**)&listB, 4 * (size_t)NT) != hipSuccess) {
+                set_error("time-sliced launches: allocation failed");
+                return MPGOS_E_MEMORY;
+            }
+        }
+        if (relaunchCountsN < relaunchLaunches + 1) {
+            MPGOS_HIP_TRY(hipStreamSynchronize(strm));
+            if (relaunchCounts) (void)hipFree(relaunchCounts);
+            relaunchCounts = nullptr; relaunchCountsN = 0;
+            if (hipMalloc((void**)&relaunchCounts, sizeof(unsigned) * (size_t)(relaunchLaunches + 1)) != hipSuccess) {
+                set_error("relaunch form: allocation failed");
+                return MPGOS_E_MEMORY;
+            }
+            relaunchCountsN = relaunchLaunches + 1;
+        }
+        return MPGOS_OK;
+    }
+
     int enqueue(const Args<Sys, P, CP>& a, KernelFn fn, int grid)
     {
+        if (kernelMode == MPGOS_KERNEL_RELAUNCH) return enqueue_relaunch(a, fn, grid);
         if (kernelMode == MPGOS_KERNEL_PERSISTENT)
             hipLaunchKernelGGL(reset_queue_kernel<0>, dim3(1), dim3(64), 0, strm, queue);
         hipLaunchKernelGGL(fn, dim3(grid), dim3(block()), dyn_lds(), strm, a);
@@ -611,6 +675,9 @@ public:
         if (ordered()) {
             if (int rc = ensure_order()) return rc;
         }
+        if (kernelMode == MPGOS_KERNEL_RELAUNCH) {
+            if (int rc = ensure_slice_buffers()) return rc;
+        }
         Args<Sys, P, CP> a = make_args();
         if (ordered()) a.list = order;
         KernelFn fn = kernel();
@@ -621,7 +688,8 @@ public:
         if (!graphCapture) {
             if (int rc = enqueue(a, fn, grid)) return rc;
         } else {
-            const int key = (algorithm << 8) | (kernelMode << 4) | variant;
+            const long long key = ((long long)relaunchSteps << 32) | ((long long)relaunchLaunches << 16) | (algorithm << 8) |
+                                  (kernelMode << 4) | variant;
             const unsigned char* ab = reinterpret_cast<const unsigned char*>(&a);
             const bool same = graphExec && key == graphKey && graphArgs.size() == sizeof a &&
                               std::memcmp(graphArgs.data(), ab, sizeof a) == 0 && graphGrid == grid &&
@@ -672,13 +740,7 @@ public:
         if (budget < 1 || budget > 0x7fffffffLL) { set_error("solve_slice: step budget must be in 1..2^31-1"); return MPGOS_E_ARG; }
         if (graphCapture) { set_error("solve_slice: not available with GraphCapture"); return MPGOS_E_STATE; }
         MPGOS_HIP_TRY(hipSetDevice(device));
-        if (!saved) {
-            if (hipMalloc((void**)&saved, lane_words() * 4 * (size_t)NT) != hipSuccess || hipMalloc((void**)&phase, 4 * (size_t)NT) != hipSuccess ||
-                hipMalloc((void**)&listA, 4 * (size_t)NT) != hipSuccess || hipMalloc((void**)&listB, 4 * (size_t)NT) != hipSuccess) {
-                set_error("solve_slice: allocation failed");
-                return MPGOS_E_MEMORY;
-            }
-        }
+        if (int rc = ensure_slice_buffers()) return rc;
         if (!sliceActive) {
             MPGOS_HIP_TRY(hipMemsetAsync(phase, 0, 4 * (size_t)NT, strm));
             sliceList = nullptr;

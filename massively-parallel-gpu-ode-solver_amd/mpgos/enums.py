@@ -64,11 +64,14 @@ class ListOfSolverOptions(enum.IntEnum):
     PersistentRoundSteps = 19
     QueueOrder = 20
     PersistentSoloInstances = 21
+    RelaunchSteps = 22
+    RelaunchLaunches = 23
 
 
 class KernelMode(enum.IntEnum):
     PLAIN = 0
     PERSISTENT = 1
+    RELAUNCH = 2
 
 
 class Status(enum.IntFlag):

@@ -4,8 +4,10 @@ DeviceSnapshot keeps device-side copies of a solver's input buffers so repeated
 Solve() calls start from identical inputs without host traffic (inputs stay
 resident in HBM).  autotune() times candidate launch configurations on the
 caller's real workload and returns the fastest:
-  kernel form      plain (one lane per instance) or persistent (work queue
-                   with lane compaction),
+  kernel form      plain (one lane per instance), persistent (work queue
+                   with lane compaction) or relaunch (one lane per instance,
+                   relaunched every RelaunchSteps steps with the unfinished
+                   instances repacked),
   ThreadsPerBlock  64 .. 512,
   LaunchVariant    register budget (waves-per-EU hint 1, 2, 4, 8),
   PersistentRoundSteps  attempted steps between compactions.
@@ -92,6 +94,8 @@ def apply(solver, cfg):
     solver.SolverOption(O.PersistentWavesPerSimd, cfg.get("waves_per_simd", 0))
     solver.SolverOption(O.QueueOrder, cfg.get("order", 0))
     solver.SolverOption(O.PersistentSoloInstances, cfg.get("solo", 0))
+    solver.SolverOption(O.RelaunchSteps, cfg.get("relaunch", 300))
+    solver.SolverOption(O.RelaunchLaunches, cfg.get("launches", 16))
 
 
 def time_config(solver, snap, cfg, repeats=2, warmup=1):
@@ -117,7 +121,9 @@ def default_candidates():
     plain = [{"mode": 0, "block": b, "variant": v} for b, v in itertools.product((64, 128, 256), (0, 2))]
     pers = [{"mode": 1, "block": b, "variant": v, "round": r}
             for b, v, r in itertools.product((256, 512), (0, 2), (8, 32, 128))]
-    return plain + pers
+    relaunch = [{"mode": 2, "block": b, "relaunch": r, "launches": 32}
+                for b, r in itertools.product((64, 256), (100, 200, 400))]
+    return plain + pers + relaunch
 
 
 def autotune(solver, candidates=None, repeats=2, cache=None, key=None, log=None):

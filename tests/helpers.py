@@ -6,7 +6,8 @@ from mpgos import workloads as W
 
 
 def run_gpu(spec, precision=Precision.DOUBLE, kernel_mode=0, block=None, variant=0, graph=0, solves=1,
-            waves_per_simd=0, slice_steps=0, queue_order=0, round_steps=0, solo=0):
+            waves_per_simd=0, slice_steps=0, queue_order=0, round_steps=0, solo=0, relaunch_steps=0,
+            relaunch_launches=None):
     s = ProblemSolver(spec["system"], spec["nt"], spec["algorithm"], precision, ndo=spec.get("ndo", 0))
     try:
         W.load(s, spec)
@@ -16,6 +17,10 @@ def run_gpu(spec, precision=Precision.DOUBLE, kernel_mode=0, block=None, variant
         s.SolverOption(O.PersistentWavesPerSimd, waves_per_simd)
         s.SolverOption(O.QueueOrder, queue_order)
         s.SolverOption(O.PersistentSoloInstances, solo)
+        if relaunch_steps:
+            s.SolverOption(O.RelaunchSteps, relaunch_steps)
+        if relaunch_launches is not None:
+            s.SolverOption(O.RelaunchLaunches, relaunch_launches)
         if round_steps:
             s.SolverOption(O.PersistentRoundSteps, round_steps)
         if block:

@@ -77,11 +77,11 @@ import scenarios as S  # noqa: E402
 _ALL = S.all_scenarios()
 
 
-@pytest.mark.parametrize("mode", [0, 1], ids=["plain", "persistent"])
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=["plain", "persistent", "relaunch"])
 @pytest.mark.parametrize("name,system,spec", _ALL, ids=[x[0] for x in _ALL])
 def test_scenarios_bitwise(gpu, oracle, name, system, spec, mode):
     ref = oracle.run(system, spec)
-    out = run_gpu(spec, kernel_mode=mode)
+    out = run_gpu(spec, kernel_mode=mode, relaunch_steps=7 if mode == 2 else 0, relaunch_launches=3 if mode == 2 else None)
     keys = ("X", "T", "TD", "cPAR", "ACC", "ACCi", "nacc", "nrej", "status", "nev", "DOidx", "DOt", "DOx",
             "evlog", "evlogidx")
     assert_bitwise(out, ref, keys=keys, label=f"{name}/{mode}")
@@ -91,7 +91,7 @@ def test_scenarios_bitwise(gpu, oracle, name, system, spec, mode):
 def test_launch_shapes_do_not_change_results(gpu, oracle, block, variant):
     spec = W.config1_lorenz(nt=3000)
     ref = oracle.run("lorenz", spec)
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         out = run_gpu(spec, kernel_mode=mode, block=block, variant=variant)
         assert_bitwise(out, ref, label=f"lorenz block={block} variant={variant} mode={mode}")
 
@@ -121,6 +121,25 @@ def test_persistent_longest_first_queue(gpu, oracle, slice_steps, solo):
     assert_bitwise(out, ref, keys=("X", "T", "nacc", "nrej", "status", "nev", "evlog", "evlogidx"), label="longest first")
 
 
+@pytest.mark.parametrize("steps,launches", [(997, 40), (500, 2), (1, 0)], ids=["all_budgeted", "final_unbudgeted", "one_launch"])
+def test_relaunch_form_is_bitwise(gpu, oracle, steps, launches):
+    """Relaunch form: budgeted launches with the unfinished instances repacked
+    (list length on the device), then one unbudgeted launch; events, logs and
+    a ragged active count included, repeated Solves continue."""
+    full = W.config2_duffing_uneven(nt=131072, evcap=256)
+    idx = np.argsort(full["TD"][1])[:700]
+    spec = W.subset(full, idx)
+    spec["n_active"] = 650
+    print(f"relaunch steps={steps} launches={launches}", flush=True)
+    ref = oracle.run("duffing_eps", spec)
+    outs = run_gpu(spec, kernel_mode=2, relaunch_steps=steps, relaunch_launches=launches, solves=2)
+    keys = ("X", "T", "nacc", "nrej", "status", "nev", "evlog", "evlogidx")
+    assert_bitwise(outs[0], ref, keys=keys, label="relaunch")
+    cur = dict(spec)
+    cur["X"], cur["T"] = ref["X"], ref["T"]
+    assert_bitwise(outs[1], oracle.run("duffing_eps", cur), keys=keys, label="relaunch second solve")
+
+
 def test_queue_order_option_is_checked(gpu):
     from mpgos import ListOfSolverOptions as O, MPGOSError, ProblemSolver
     s = ProblemSolver("lorenz", 64, 1)
@@ -135,11 +154,13 @@ def test_graph_replays_match_eager(gpu, oracle):
     from mpgos import ListOfSolverOptions as O, ListOfVariables as V, ProblemSolver
     spec = W.config1_lorenz(nt=2048)
     ref = oracle.run("lorenz", spec)
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         s = ProblemSolver("lorenz", spec["nt"], Algorithms.RKCK45)
         try:
             W.load(s, spec)
             s.SolverOption(O.KernelMode, mode)
+            s.SolverOption(O.RelaunchSteps, 200)         # relaunch form: several launches in the graph
+            s.SolverOption(O.RelaunchLaunches, 4)
             s.SolverOption(O.GraphCapture, 1)
             for rep in range(3):
                 W.load(s, spec)                      # eager uploads between replays
