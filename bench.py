@@ -172,6 +172,9 @@ def per_thread(args, mpgos, W, pyoracle):
         if sl and args.solo:       # longest interval first, the longest instances alone in one-wave workgroups
             pcfg = {"mode": 1, "block": 64, "round": 256, "waves_per_simd": 1, "order": 1, "solo": args.solo}
         forms["persistent"] = time_form(mpgos, spec, pcfg, args.steps, args.warmup, prec, sl)
+    if args.form in ("auto", "relaunch") and not sl:
+        forms["relaunch"] = time_form(mpgos, spec, {"mode": 2, "block": 256, "relaunch": 100, "launches": 32},
+                                      args.steps, args.warmup, prec)
     if args.form == "auto" and not args.no_tune and not sl:
         best_cfg = tune(mpgos, spec, prec)
         forms["autotuned"] = time_form(mpgos, spec, best_cfg, args.steps, args.warmup, prec)
@@ -222,6 +225,8 @@ def per_thread(args, mpgos, W, pyoracle):
                          "instance_steps_per_s": accepted / (v["wall_ms_per_step"] * 1e-3)} for k, v in forms.items()}
     if "plain" in forms and "persistent" in forms:
         line["persistent_over_plain"] = forms["plain"]["kernel_ms"] / forms["persistent"]["kernel_ms"]
+    if "plain" in forms and "relaunch" in forms:
+        line["relaunch_over_plain"] = forms["plain"]["kernel_ms"] / forms["relaunch"]["kernel_ms"]
     return line
 
 
@@ -491,7 +496,8 @@ def main():
     ap.add_argument("--workload", default="lorenz",
                     choices=["lorenz", "duffing_rk4", "duffing_uneven", "duffing_adjoint", "ring"])
     ap.add_argument("--nt", type=int, default=0, help="instances (systems for ring); 0 = the workload's size")
-    ap.add_argument("--form", default="auto", help="auto | plain | persistent (per-thread); auto | mfma | lane (ring)")
+    ap.add_argument("--form", default="auto",
+                    help="auto | plain | persistent | relaunch (per-thread); auto | mfma | lane (ring)")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "f32k"])
     ap.add_argument("--algorithm", default="rk4", choices=["rk4", "rkck45"], help="duffing_adjoint only")
     ap.add_argument("--checkpoint-interval", type=int, default=0,
