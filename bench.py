@@ -86,7 +86,42 @@ def sliced_solve(s, budget, label):
             return total
 
 
-def time_form(mpgos, spec, cfg, steps, warmup, precision=0, slice_steps=0):
+def watched_solve(s, label, every=20.0):
+    """One Solve() that may run for minutes, with progress from the work
+    itself: each instance writes its accepted-step count when it finishes, so
+    while the Solve runs a second stream copies those counts (cleared before
+    the Solve) and the number of finished instances is reported.  Returns the
+    Solve's kernel time in ms."""
+    from mpgos.enums import ListOfVariables as V
+    from mpgos.tuning import hip
+    h = hip()
+    h.hipStreamQuery.argtypes = [ctypes.c_void_p]
+    h.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    h.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    h.hipMemsetAsync.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p]
+    ptr, nbytes = s.device_ptr(V.AcceptedSteps)
+    counts = np.empty(nbytes // 8, np.int64)
+    side = ctypes.c_void_p()
+    if h.hipStreamCreate(ctypes.byref(side)) != 0 or h.hipMemsetAsync(ptr, 0, nbytes, s.stream()) != 0:
+        raise RuntimeError("watched_solve: stream or memset failed")
+    try:
+        s.Solve()
+        t0 = last = time.perf_counter()
+        while h.hipStreamQuery(s.stream()) != 0:
+            time.sleep(0.5)
+            if time.perf_counter() - last >= every:
+                last = time.perf_counter()
+                if h.hipMemcpyAsync(counts.ctypes.data, ptr, nbytes, 2, side) != 0 or h.hipStreamSynchronize(side) != 0:
+                    raise RuntimeError("watched_solve: progress copy failed")
+                log(f"{label}: {np.count_nonzero(counts)} of {counts.size} instances finished after {last - t0:.0f} s")
+        if h.hipStreamSynchronize(s.stream()) != 0:
+            raise RuntimeError("watched_solve: Solve failed")
+    finally:
+        h.hipStreamDestroy(side)
+    return s.last_solve_ms()
+
+
+def time_form(mpgos, spec, cfg, steps, warmup, precision=0, slice_steps=0, watch=False):
     """K timed steps (device-side input reset + Solve) of one launch configuration."""
     from mpgos import workloads as W
     from mpgos.tuning import DeviceSnapshot, apply
@@ -95,7 +130,7 @@ def time_form(mpgos, spec, cfg, steps, warmup, precision=0, slice_steps=0):
         W.load(s, spec)
         apply(s, cfg)
         snap = DeviceSnapshot(s)
-        for _ in range(warmup):
+        for _ in range(warmup if not watch else 0):
             snap.restore()
             if slice_steps:
                 snap.sync()
@@ -110,6 +145,10 @@ def time_form(mpgos, spec, cfg, steps, warmup, precision=0, slice_steps=0):
             if slice_steps:
                 snap.sync()
                 kernel_ms.append(sliced_solve(s, slice_steps, f"{cfg}"))
+                continue
+            if watch:
+                snap.sync()
+                kernel_ms.append(watched_solve(s, f"{cfg}"))
                 continue
             s.Solve()
             snap.sync()
@@ -165,6 +204,11 @@ def per_thread(args, mpgos, W, pyoracle):
         desc = {"system": "duffing_eps", "algorithm": "RKCK45", "rtol": 1e-9, "atol": 1e-9, "events": "x'=0, 256 logged"}
     forms = {}
     sl = args.slice_steps if args.workload == "duffing_uneven" else 0
+    if sl and args.form == "single":        # the reference's form: one plain launch for the whole batch
+        forms["single"] = time_form(mpgos, spec, {"mode": 0, "block": 64}, args.steps, 0, prec, 0, watch=True)
+    if sl and args.form == "relaunch":      # device-chained relaunches, budget --slice-steps per launch
+        rcfg = {"mode": 2, "block": 64, "relaunch": sl, "launches": 64}
+        forms["relaunch"] = time_form(mpgos, spec, rcfg, args.steps, 0, prec, 0, watch=True)
     if args.form in ("auto", "plain"):
         forms["plain"] = time_form(mpgos, spec, {"mode": 0, "block": 64}, args.steps, args.warmup, prec, sl)
     if args.form in ("auto", "persistent"):
@@ -497,7 +541,8 @@ def main():
                     choices=["lorenz", "duffing_rk4", "duffing_uneven", "duffing_adjoint", "ring"])
     ap.add_argument("--nt", type=int, default=0, help="instances (systems for ring); 0 = the workload's size")
     ap.add_argument("--form", default="auto",
-                    help="auto | plain | persistent | relaunch (per-thread); auto | mfma | lane (ring)")
+                    help="auto | plain | persistent | relaunch (per-thread; duffing_uneven also: single = one "
+                         "plain launch, relaunch = device-chained launches of --slice-steps); auto | mfma | lane (ring)")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "f32k"])
     ap.add_argument("--algorithm", default="rk4", choices=["rk4", "rkck45"], help="duffing_adjoint only")
     ap.add_argument("--checkpoint-interval", type=int, default=0,
