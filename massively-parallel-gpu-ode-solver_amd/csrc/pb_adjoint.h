@@ -623,7 +623,7 @@ __device__ void pb_replay_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sy
             }
         }
         lds_barrier();
-        mfma_contract<NC>(a.cMat, cptB, coup, UPS, SPB);
+        mfma_contract44<NC>(a.cMat, cptB, coup, UPS, SPB);
         lds_barrier();
         if (run) {
 #pragma unroll
@@ -824,7 +824,7 @@ __device__ void pb_sweep_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys
                 }
             }
             __syncthreads();
-            mfma_contract<NC, 4>(A.cMatT, cptB, coup, UPS, SPB);
+            mfma_contract44<NC, 4>(A.cMatT, cptB, coup, UPS, SPB);
             __syncthreads();
 #pragma unroll
             for (int p = 0; p < PAIRS; p++) {
