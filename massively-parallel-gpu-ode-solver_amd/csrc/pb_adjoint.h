@@ -824,7 +824,7 @@ __device__ void pb_sweep_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys
                 }
             }
             __syncthreads();
-            mfma_contract44<NC, 4>(A.cMatT, cptB, coup, UPS, SPB);
+            mfma_contract44<NC>(A.cMatT, cptB, coup, UPS, SPB);
             __syncthreads();
 #pragma unroll
             for (int p = 0; p < PAIRS; p++) {

@@ -273,8 +273,8 @@ __device__ __forceinline__ void mfma_contract(const double* M, const double* cpt
 // as a fused multiply-add chain in k order.  The four blocks take the four
 // 4-row slices of a 16-unit tile against the same 4 systems, so the A operand
 // is the 16 x 16 x 4 form's and no B column is zero: two chains per tile (systems
-// 0..3 and 4..7) do the work of one 16 x 16 x 4 chain with half its columns
-// unused.  Output (unit 4 ((lane / 4) % 4) + lane / 16 of the tile, system
+// 0..3 and 4..7, independent, so one tile per pass keeps two in flight) do the
+// work of one 16 x 16 x 4 chain with half its columns unused.  Output (unit 4 ((lane / 4) % 4) + lane / 16 of the tile, system
 // lane % 4 (+ 4)) in one register per chain.  Bit-identical to mfma_contract.
 template <int NC, int G = 8>
 __device__ __forceinline__ void mfma_contract44(const double* M, const double* cptB, double* coup, int UPS, int SPB)
@@ -285,40 +285,30 @@ __device__ __forceinline__ void mfma_contract44(const double* M, const double* c
     const int r16 = l & 15, k4 = l >> 4, sys = l & 3;
     const int drow = 4 * ((l >> 2) & 3) + (l >> 4);
     const int blocks = UPS >> 2;
-    for (int t0 = threadIdx.x >> 6; t0 < tiles; t0 += 2 * nw) {
-        const int t1 = t0 + nw;
-        const bool two = t1 < tiles;                      // wave-uniform
+    // one 16-unit tile per pass; its two system halves are two independent chains
+    for (int t = threadIdx.x >> 6; t < tiles; t += nw) {
 #pragma unroll
         for (int i = 0; i < NC; i++) {
-            const double* A0 = M + (size_t)i * UPS * UPS + t0 * 16 + r16 + (size_t)k4 * UPS;
-            const double* A1 = M + (size_t)i * UPS * UPS + (two ? t1 : t0) * 16 + r16 + (size_t)k4 * UPS;
+            const double* Ai = M + (size_t)i * UPS * UPS + t * 16 + r16 + (size_t)k4 * UPS;
             const double* Bi = cptB + (size_t)i * UPS * 16 + k4 * 16 + sys;
-            double acc00 = 0.0, acc01 = 0.0, acc10 = 0.0, acc11 = 0.0;   // [tile][systems 0..3 | 4..7]
+            double acc0 = 0.0, acc1 = 0.0;              // systems 0..3, 4..7
             for (int c0 = 0; c0 < blocks; c0 += G) {
-                double a0[G], a1[G], b0[G], b1[G];
+                double a[G], b0[G], b1[G];
 #pragma unroll
                 for (int d = 0; d < G; d++) {
-                    a0[d] = A0[(size_t)(c0 + d) * 4 * UPS];
-                    a1[d] = A1[(size_t)(c0 + d) * 4 * UPS];
+                    a[d] = Ai[(size_t)(c0 + d) * 4 * UPS];
                     b0[d] = Bi[(c0 + d) * 64];
                     b1[d] = Bi[(c0 + d) * 64 + 4];
                 }
 #pragma unroll
                 for (int d = 0; d < G; d++) {
-                    acc00 = __builtin_amdgcn_mfma_f64_4x4x4f64(a0[d], b0[d], acc00, 0, 0, 0);
-                    acc01 = __builtin_amdgcn_mfma_f64_4x4x4f64(a0[d], b1[d], acc01, 0, 0, 0);
-                    acc10 = __builtin_amdgcn_mfma_f64_4x4x4f64(a1[d], b0[d], acc10, 0, 0, 0);
-                    acc11 = __builtin_amdgcn_mfma_f64_4x4x4f64(a1[d], b1[d], acc11, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b0[d], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b1[d], acc1, 0, 0, 0);
                 }
             }
-            double* c0p = coup + ((size_t)i * UPS + t0 * 16 + drow) * SPB;
-            if (sys < SPB) c0p[sys] = acc00;
-            if (sys + 4 < SPB) c0p[sys + 4] = acc01;
-            if (two) {
-                double* c1p = coup + ((size_t)i * UPS + t1 * 16 + drow) * SPB;
-                if (sys < SPB) c1p[sys] = acc10;
-                if (sys + 4 < SPB) c1p[sys + 4] = acc11;
-            }
+            double* cp = coup + ((size_t)i * UPS + t * 16 + drow) * SPB;
+            if (sys < SPB) cp[sys] = acc0;
+            if (sys + 4 < SPB) cp[sys + 4] = acc1;
         }
     }
 }
