@@ -28,18 +28,15 @@
 //    coupling matrix from L2 (or LDS), coupling terms double-buffered in LDS
 //    (one barrier per stage);
 //  * dense contraction (full matrix, FP64, UPS a multiple of 32 up to 256, 8
-//    systems per workgroup, 4 pairs per lane): the products C . [CPT_s0 .. CPT_s7] of all 8 systems run on
-//    v_mfma_f64_16x16x4f64.  Wave w owns unit tiles w, w + waves, ...  (two
-//    chains at a time); A is the
-//    16 x 4 tile M[16 t + i][4 c + k] (lane i + 16 k, from L2), B the 4 x 16
-//    tile CPT[4 c + k][system j] (lane j + 16 k, from LDS, columns 8..15
-//    zero), and the accumulator chains the 64 column blocks in order.  The
-//    instruction accumulates its four products as a chain of fused
-//    multiply-adds in k order, so the result equals the sequential
-//    fma(M[u][col], CPT[col], v) over col = 0 .. UPS-1 of the lane-per-pair
-//    form bit for bit.  Output element (row i, column j) sits in lane
-//    j + 16 (i mod 4), register i / 4; it goes through LDS to the lane that
-//    owns the pair (two barriers per stage).
+//    systems per workgroup, 4 pairs per lane): the products C . [CPT_s0 ..
+//    CPT_s7] of all 8 systems run on v_mfma_f64_4x4x4f64 (mfma_contract44:
+//    four 4 x 4 x 4 blocks per instruction take the four 4-row slices of a
+//    16-unit tile against 4 systems; two chains per tile cover the 8
+//    systems).  The instruction accumulates its four products as a chain of
+//    fused multiply-adds in k order, so chaining the column blocks in order
+//    equals the sequential fma(M[u][col], CPT[col], v) over col = 0 .. UPS-1
+//    of the lane-per-pair form bit for bit.  Results go through LDS to the
+//    lane that owns the pair (two barriers per stage).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -215,67 +212,21 @@ struct Unit {
     }
 };
 
-typedef double mfma_d4 __attribute__((ext_vector_type(4)));
-
 // Dense contraction of the coupling terms of the block's systems: for every
-// coupling i, unit u and system s < SPB,
+// coupling i, unit u and system s < SPB (at most 8),
 //   coup[(i * UPS + u) * SPB + s] = sum_col M_i[u + col * UPS] * cptB[(i * UPS + col) * 16 + s]
-// accumulated in column order.  Called by every lane of the workgroup
-// (v_mfma needs all 64 lanes of a wave: blockDim.x is a multiple of 64); UPS is
-// a multiple of 16.
-template <int NC, int G = 8>    // G: column blocks per load group (UPS % (4 G) == 0)
-__device__ __forceinline__ void mfma_contract(const double* M, const double* cptB, double* coup, int UPS, int SPB)
-{
-    const int l = threadIdx.x & 63;
-    const int nw = blockDim.x >> 6;
-    const int tiles = UPS >> 4;
-    const int i16 = l & 15, k4 = l >> 4;
-    const int blocks = UPS >> 2;
-    // two unit tiles per pass (two independent accumulation chains per wave)
-    for (int t0 = threadIdx.x >> 6; t0 < tiles; t0 += 2 * nw) {
-        const int t1 = t0 + nw;
-        const bool two = t1 < tiles;                      // wave-uniform
-#pragma unroll
-        for (int i = 0; i < NC; i++) {
-            const double* A0 = M + (size_t)i * UPS * UPS + t0 * 16 + i16 + (size_t)k4 * UPS;
-            const double* A1 = M + (size_t)i * UPS * UPS + (two ? t1 : t0) * 16 + i16 + (size_t)k4 * UPS;
-            const double* Bi = cptB + (size_t)i * UPS * 16 + k4 * 16 + i16;
-            mfma_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-            for (int c0 = 0; c0 < blocks; c0 += G) {
-                double a0[G], a1[G], b[G];
-#pragma unroll
-                for (int d = 0; d < G; d++) {
-                    a0[d] = A0[(size_t)(c0 + d) * 4 * UPS];
-                    a1[d] = A1[(size_t)(c0 + d) * 4 * UPS];
-                    b[d] = Bi[(c0 + d) * 64];
-                }
-#pragma unroll
-                for (int d = 0; d < G; d++) {
-                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[d], b[d], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[d], b[d], acc1, 0, 0, 0);
-                }
-            }
-            if (i16 < SPB) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    coup[((size_t)i * UPS + t0 * 16 + k4 + 4 * r) * SPB + i16] = acc0[r];
-                    if (two) coup[((size_t)i * UPS + t1 * 16 + k4 + 4 * r) * SPB + i16] = acc1[r];
-                }
-            }
-        }
-    }
-}
-
-// The same contraction on v_mfma_f64_4x4x4f64, which computes four independent
+// accumulated in column order, on v_mfma_f64_4x4x4f64 (called by every lane of
+// the workgroup: the instruction needs all 64 lanes of a wave; UPS is a
+// multiple of 16).  The instruction computes four independent
 // 4 x 4 x 4 products per instruction (block b = lanes with (lane / 4) % 4 == b:
 // A[i][k] in lane i + 4b + 16k, B[k][j] in lane j + 4b + 16k, D[i][j] in lane
-// j + 4b + 16i) and, like the 16 x 16 x 4 form, accumulates its four products
-// as a fused multiply-add chain in k order.  The four blocks take the four
+// j + 4b + 16i) and accumulates its four products as a fused multiply-add
+// chain in k order (both probed on the MI355X).  The four blocks take the four
 // 4-row slices of a 16-unit tile against the same 4 systems, so the A operand
-// is the 16 x 16 x 4 form's and no B column is zero: two chains per tile (systems
-// 0..3 and 4..7, independent, so one tile per pass keeps two in flight) do the
-// work of one 16 x 16 x 4 chain with half its columns unused.  Output (unit 4 ((lane / 4) % 4) + lane / 16 of the tile, system
-// lane % 4 (+ 4)) in one register per chain.  Bit-identical to mfma_contract.
+// is lane (row = lane % 16, column = lane / 16) and every B column is a system:
+// two chains per tile (systems 0..3 and 4..7, independent, so one tile per
+// pass keeps two in flight).  Output (unit 4 ((lane / 4) % 4) + lane / 16 of
+// the tile, system lane % 4 (+ 4)) in one register per chain.
 template <int NC, int G = 8>
 __device__ __forceinline__ void mfma_contract44(const double* M, const double* cptB, double* coup, int UPS, int SPB)
 {
