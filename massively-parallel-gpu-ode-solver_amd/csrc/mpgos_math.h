@@ -155,19 +155,25 @@ MPGOS_HD MPGOS_INLINE double exp_kernel(double r)
 }
 } // namespace detail
 
+// The range checks select the result after the polynomial instead of branching
+// around it: a wavefront then runs one instruction stream whatever its lanes'
+// arguments (no divergent paths, no saved execution masks).  The value is the
+// same as with early returns for every argument.
 MPGOS_HD MPGOS_INLINE double exp(double x)
 {
-    if (x != x) return x;
-    if (x > 709.782712893384) return from_bits(0x7ff0000000000000ull);
-    if (x < -745.1332191019412) return 0.0;
-    const double n = rint(x * detail::LOG2E);
-    double r = fma(-n, detail::LN2_HI, x);   // exact (see reduce_pio2)
+    constexpr double HI = 709.782712893384, LO = -745.1332191019412;
+    const double xc = x >= LO ? (x <= HI ? x : HI) : LO;     // NaN -> LO, replaced below
+    const double n = rint(xc * detail::LOG2E);
+    double r = fma(-n, detail::LN2_HI, xc);   // exact (see reduce_pio2)
     r = fma(-n, detail::LN2_LO, r);
     const double p = detail::exp_kernel(r);
     const int k = (int)n;
     // two-step scaling keeps 2^k representable near the overflow / subnormal ends
     const int k1 = k / 2;
-    return (p * pow2i(k1)) * pow2i(k - k1);
+    double v = (p * pow2i(k1)) * pow2i(k - k1);
+    v = x < LO ? 0.0 : v;
+    v = x > HI ? from_bits(0x7ff0000000000000ull) : v;
+    return x != x ? x : v;
 }
 
 // ---------------------------------------------------------------- log
@@ -239,31 +245,30 @@ MPGOS_HD MPGOS_INLINE void log_dd(double x, double& hi, double& lo)
 // rounding error folded back (exp(p + e) ~ exp(p) (1 + e)), so the result is
 // within a few ulp over the controller's range (x up to 1e30, y = 1/5, 1/4).
 // IEEE special cases for x in {0, inf, NaN}; negative x gives NaN unless y is
-// an integer.
+// an integer.  The general formula runs for every argument (on 1 where x is 0,
+// inf or NaN) and the special cases then override it from the lowest to the
+// highest precedence, so no lane branches.
 MPGOS_HD MPGOS_INLINE double pow(double x, double y)
 {
-    if (x != x || y != y) return x + y;
-    if (y == 0.0) return 1.0;
-    if (x == 1.0) return 1.0;
-    if (x == 0.0) return y > 0.0 ? 0.0 : from_bits(0x7ff0000000000000ull);
-    if (!isfinite(x)) {
-        if (x > 0.0) return y > 0.0 ? x : 0.0;
-        return from_bits(0x7ff8000000000000ull);
-    }
-    double sign = 1.0;
-    if (x < 0.0) {
-        if (rint(y) != y) return from_bits(0x7ff8000000000000ull);
-        const double half = 0.5 * y;
-        if (rint(half) != half) sign = -1.0;   // odd integer exponent
-        x = -x;
-    }
+    constexpr double INF = __builtin_huge_val(), QNAN = __builtin_nan("");
+    const double ax = fabs(x);
+    const bool regular = ax > 0.0 && isfinite(ax);
     double lh, ll;
-    detail::log_dd(x, lh, ll);
+    detail::log_dd(regular ? ax : 1.0, lh, ll);
     const double ph = y * lh;
     const double pl = fma(y, lh, -ph) + y * ll;
     const double e = exp(ph);
-    if (!isfinite(e) || e == 0.0) return sign * e;
-    return sign * fma(e, pl, e);
+    double r = (isfinite(e) && e != 0.0) ? fma(e, pl, e) : e;
+    // negative base: integer exponents only, the sign from the exponent's parity
+    const double half = 0.5 * y;
+    const bool odd = rint(half) != half;
+    if (x < 0.0) r = rint(y) != y ? QNAN : (odd ? -r : r);
+    if (!isfinite(x)) r = x > 0.0 ? (y > 0.0 ? x : 0.0) : QNAN;
+    if (x == 0.0) r = y > 0.0 ? 0.0 : INF;
+    if (x == 1.0) r = 1.0;
+    if (y == 0.0) r = 1.0;
+    if (x != x || y != y) r = x + y;
+    return r;
 }
 
 // ---------------------------------------------------------------- float versions

@@ -327,9 +327,9 @@ struct Lane {
             update = update && (err[i] < tol);
             R = mpgos_fmin(R, tol / err[i]);
         }
-        CP fac;
-        if (update == 1) fac = CP(0.9) * math::pow(R, CP(1.0 / 5.0));
-        else fac = CP(0.9) * math::pow(R, CP(1.0 / 4.0));
+        // one power for both outcomes: lanes of a wavefront that accept and
+        // lanes that reject then share one evaluation
+        CP fac = CP(0.9) * math::pow(R, update == 1 ? CP(1.0 / 5.0) : CP(1.0 / 4.0));
         if (!math::isfinite(fac)) isfin = 0;
 
         if (isfin == 0) {

@@ -117,3 +117,56 @@ def test_sqrt_is_correctly_rounded():
     out = host_eval("sqrt", x)
     assert np.array_equal(out.view(np.uint64), np.sqrt(x).view(np.uint64))
     assert np.isnan(host_eval("sqrt", np.array([-1.0, np.nan]))).all()
+
+
+def _pow_rules(x, y):
+    """pow's special cases, in order of precedence, then the general value
+    (mpmath on |x|, signed by the exponent's parity for a negative base)."""
+    import math
+    if math.isnan(x) or math.isnan(y):
+        return math.nan
+    if y == 0.0 or x == 1.0:
+        return 1.0
+    if x == 0.0:
+        return 0.0 if y > 0.0 else math.inf
+    if math.isinf(x):
+        return (x if y > 0.0 else 0.0) if x > 0.0 else math.nan
+    sign = 1.0
+    if x < 0.0:
+        if float(np.rint(y)) != y:
+            return math.nan
+        if float(np.rint(0.5 * y)) != 0.5 * y:
+            sign = -1.0
+    if math.isinf(y):   # |x| = 1: inf * log 1 is NaN (C99 would give 1)
+        if abs(x) == 1.0:
+            return math.nan
+        return sign * (math.inf if (abs(x) > 1.0) == (y > 0) else 0.0)
+    mpmath.mp.dps = 40
+    v = mpmath.power(mpmath.mpf(abs(x)), mpmath.mpf(y))
+    return sign * (math.inf if v > 1.8e308 else float(v))
+
+
+def test_pow_and_exp_special_cases():
+    """Every combination of special and ordinary arguments against the rules
+    above: zeros of both signs, infinities, NaN, one, negative bases with
+    integer, odd and fractional exponents, subnormals, over- and underflow."""
+    xs = [0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -1.0, 2.0, -2.0, 0.5, -0.5, 1e-310, -3.0, 1e30, 7.25]
+    ys = [0.0, -0.0, 1.0, 2.0, 3.0, -3.0, 0.5, -0.5, 0.2, 0.25, np.inf, -np.inf, np.nan, 1e300, -1e300, 2.5]
+    X, Y = np.meshgrid(np.array(xs), np.array(ys))
+    got = host_eval("pow", X.ravel(), Y.ravel())
+    for x, y, g in zip(X.ravel(), Y.ravel(), got):
+        want = _pow_rules(float(x), float(y))
+        if np.isnan(want):
+            assert np.isnan(g), (x, y, g)
+        elif want == 0.0 or np.isinf(want):
+            assert g == want, (x, y, g, want)
+        else:   # ordinary values: accuracy is pinned in the controller's range above
+            assert abs(g - want) <= max(1e-14 * abs(want), 4 * 5e-324), (x, y, g, want)
+    hi, lo = 709.782712893384, -745.1332191019412
+    xe = np.array([np.nan, np.inf, -np.inf, hi, np.nextafter(hi, np.inf), lo, np.nextafter(lo, -np.inf),
+                   -744.0, -708.5, 0.0, -0.0])
+    ge = host_eval("exp", xe)
+    assert np.isnan(ge[0]) and ge[1] == np.inf and ge[2] == 0.0
+    assert np.isfinite(ge[3]) and ge[4] == np.inf and ge[5] in (0.0, 5e-324) and ge[6] == 0.0
+    ref = mp_ref(mpmath.exp, xe[[3, 7, 8, 9, 10]])
+    assert ulp_err(ge[[3, 7, 8, 9, 10]], ref).max() <= 2.0
