@@ -378,7 +378,7 @@ const char* mpgos_pb_list_systems(void);
 
 /* ---------------------------------------------------------- math probes
  * Evaluate the library's elementary functions (0 cos, 1 sin, 2 exp, 3 log,
- * 4 pow(x, y), 5 sqrt) on the host, or on the device over device buffers, so tests can
+ * 4 pow(x, y), 5 sqrt, 6 the controllers' power pow_ctrl(x, y)) on the host, or on the device over device buffers, so tests can
  * show that both sides produce the same bits. */
 int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, long n);
 int mpgos_math_eval_device(int fn, const double* x_dev, const double* y_dev, double* out_dev, long n);

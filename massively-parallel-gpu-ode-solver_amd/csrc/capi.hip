@@ -45,6 +45,7 @@ __global__ void math_eval_kernel(int fn, const double* x, const double* y, doubl
     case 2: out[i] = math::exp(x[i]); break;
     case 3: out[i] = math::log(x[i]); break;
     case 5: out[i] = math::sqrt(x[i]); break;
+    case 6: out[i] = math::pow_ctrl(x[i], y[i]); break;
     default: out[i] = math::pow(x[i], y[i]); break;
     }
 }
@@ -278,7 +279,7 @@ void* mpgos_pb_stream(mpgos_solver* h)
 
 int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, long n)
 {
-    if (fn < 0 || fn > 5 || !x || !out || (fn == 4 && !y)) { set_error("mpgos_math_eval_host: bad arguments"); return MPGOS_E_ARG; }
+    if (fn < 0 || fn > 6 || !x || !out || ((fn == 4 || fn == 6) && !y)) { set_error("mpgos_math_eval_host: bad arguments"); return MPGOS_E_ARG; }
     for (long i = 0; i < n; i++) {
         switch (fn) {
         case 0: out[i] = mpgos::math::cos(x[i]); break;
@@ -286,6 +287,7 @@ int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, 
         case 2: out[i] = mpgos::math::exp(x[i]); break;
         case 3: out[i] = mpgos::math::log(x[i]); break;
         case 5: out[i] = mpgos::math::sqrt(x[i]); break;
+        case 6: out[i] = mpgos::math::pow_ctrl(x[i], y[i]); break;
         default: out[i] = mpgos::math::pow(x[i], y[i]); break;
         }
     }
@@ -294,7 +296,7 @@ int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, 
 
 int mpgos_math_eval_device(int fn, const double* x_dev, const double* y_dev, double* out_dev, long n)
 {
-    if (fn < 0 || fn > 5 || !x_dev || !out_dev || (fn == 4 && !y_dev) || n < 0) {
+    if (fn < 0 || fn > 6 || !x_dev || !out_dev || ((fn == 4 || fn == 6) && !y_dev) || n < 0) {
         set_error("mpgos_math_eval_device: bad arguments");
         return MPGOS_E_ARG;
     }

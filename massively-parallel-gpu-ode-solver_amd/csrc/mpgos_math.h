@@ -271,6 +271,30 @@ MPGOS_HD MPGOS_INLINE double pow(double x, double y)
     return r;
 }
 
+// pow for the error controllers' R^(1/5) and R^(1/4): R = min(tol/err) is
+// >= 0 or NaN.  On the domain x >= 0, +inf or NaN and 0 < y <= 1/4 it returns
+// pow(x, y)'s bits: |y log x| < 187, so exp neither overflows nor
+// underflows and the scaling is one exact multiply; x = 1 gives exactly 1
+// through the general formula.  Without pow's range selections and
+// negative-base handling the controller's step is ~35 instructions shorter.
+MPGOS_HD MPGOS_INLINE double pow_ctrl(double x, double y)
+{
+    constexpr double MAXF = 0x1.fffffffffffffp+1023;
+    const bool regular = x > 0.0 && x <= MAXF;
+    double lh, ll;
+    detail::log_dd(regular ? x : 1.0, lh, ll);
+    const double ph = y * lh;
+    const double pl = fma(y, lh, -ph) + y * ll;
+    const double n = rint(ph * detail::LOG2E);
+    double r = fma(-n, detail::LN2_HI, ph);
+    r = fma(-n, detail::LN2_LO, r);
+    const double e = detail::exp_kernel(r) * pow2i((int)n);
+    double v = fma(e, pl, e);
+    v = x == 0.0 ? 0.0 : v;
+    v = x > MAXF ? x : v;
+    return x != x ? x + y : v;
+}
+
 // ---------------------------------------------------------------- float versions
 // Used by the FP32 stepping mode: computed through the double routines and
 // rounded once, so float results are deterministic and within 1 ulp.
@@ -279,6 +303,7 @@ MPGOS_HD MPGOS_INLINE float sin(float x) { return (float)sin((double)x); }
 MPGOS_HD MPGOS_INLINE float exp(float x) { return (float)exp((double)x); }
 MPGOS_HD MPGOS_INLINE float log(float x) { return (float)log((double)x); }
 MPGOS_HD MPGOS_INLINE float pow(float x, float y) { return (float)pow((double)x, (double)y); }
+MPGOS_HD MPGOS_INLINE float pow_ctrl(float x, float y) { return (float)pow_ctrl((double)x, (double)y); }
 MPGOS_HD MPGOS_INLINE float sqrt(float x) { return (float)sqrt((double)x); }   // exact: double has > 2 * 24 + 2 bits
 
 } // namespace math

@@ -558,7 +558,7 @@ __device__ void pb_body(const Args<Sys, P>& a)
             __syncthreads();
             if (leader && my->term == 0) {
                 const P R = from_order_bits<P>(my->ratioBits);
-                P fac = P(0.8) * math::pow(R, my->update == 1 ? P(1.0 / 5.0) : P(1.0 / 4.0));
+                P fac = P(0.8) * math::pow_ctrl(R, my->update == 1 ? P(1.0 / 5.0) : P(1.0 / 4.0));
                 if (!math::isfinite(fac)) my->isfin = 0;
                 if (my->isfin == 0) {
                     fac = a.shrink;

@@ -121,8 +121,8 @@ def default_candidates():
     plain = [{"mode": 0, "block": b, "variant": v} for b, v in itertools.product((64, 128, 256), (0, 2))]
     pers = [{"mode": 1, "block": b, "variant": v, "round": r}
             for b, v, r in itertools.product((256, 512), (0, 2), (8, 32, 128))]
-    relaunch = [{"mode": 2, "block": b, "relaunch": r, "launches": 32}
-                for b, r in itertools.product((64, 256), (100, 200, 400))]
+    relaunch = [{"mode": 2, "block": b, "variant": v, "relaunch": r, "launches": 32}
+                for b, v, r in itertools.product((64, 256), (0, 2), (50, 100, 200, 400))]
     return plain + pers + relaunch
 
 

@@ -200,6 +200,9 @@ def test_relief_valve_continuation_solves(gpu, oracle):
         assert (outs[-1]["ACCi"][0] == 1).sum() > 200         # most instances ended at a maximum
 
 
+SPECIAL = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, 5e-324, 1e-310, 800.0, -800.0, 709.78, -745.0])
+
+
 def test_device_math_matches_host_bitwise(gpu):
     import ctypes
     from test_math import FN, host_eval
@@ -215,14 +218,19 @@ def test_device_math_matches_host_bitwise(gpu):
     try:
         L = gpu.lib()
         for name, fn in FN.items():
-            arg = np.ascontiguousarray(np.abs(x) if name in ("log", "pow", "sqrt") else (np.clip(x, -700, 700) if name == "exp" else x))
+            arg = np.abs(x) if name in ("log", "pow", "pow_ctrl", "sqrt") else (np.clip(x, -700, 700) if name == "exp" else x)
+            if name in ("exp", "log", "pow", "pow_ctrl", "sqrt"):     # special arguments at the end
+                arg = np.concatenate([arg[:-len(SPECIAL)], SPECIAL])
+            arg = np.ascontiguousarray(arg)
             assert hip.hipMemcpy(bufs[0], arg.ctypes.data, nbytes, 1) == 0
             assert hip.hipMemcpy(bufs[1], y.ctypes.data, nbytes, 1) == 0
             assert L.mpgos_math_eval_device(fn, bufs[0], bufs[1], bufs[2], arg.size) == 0
             dev = np.empty_like(arg)
             assert hip.hipMemcpy(dev.ctypes.data, bufs[2], nbytes, 2) == 0
-            host = host_eval(name, arg, y if name == "pow" else None)
-            assert np.array_equal(dev.view(np.uint64), host.view(np.uint64)), name
+            host = host_eval(name, arg, y if name in ("pow", "pow_ctrl") else None)
+            both_nan = np.isnan(dev) & np.isnan(host)
+            assert np.array_equal(dev.view(np.uint64)[~both_nan], host.view(np.uint64)[~both_nan]), name
+            assert np.array_equal(np.isnan(dev), np.isnan(host)), name
     finally:
         for b in bufs:
             hip.hipFree(b)

@@ -13,7 +13,7 @@ import pytest
 
 from mpgos import lib
 
-FN = {"cos": 0, "sin": 1, "exp": 2, "log": 3, "pow": 4, "sqrt": 5}
+FN = {"cos": 0, "sin": 1, "exp": 2, "log": 3, "pow": 4, "sqrt": 5, "pow_ctrl": 6}
 
 
 def host_eval(name, x, y=None):
@@ -170,3 +170,18 @@ def test_pow_and_exp_special_cases():
     assert np.isfinite(ge[3]) and ge[4] == np.inf and ge[5] in (0.0, 5e-324) and ge[6] == 0.0
     ref = mp_ref(mpmath.exp, xe[[3, 7, 8, 9, 10]])
     assert ulp_err(ge[[3, 7, 8, 9, 10]], ref).max() <= 2.0
+
+
+def test_controller_power_equals_pow_on_its_domain():
+    """pow_ctrl (the controllers' R^(1/5), R^(1/4)) returns pow's bits for
+    x >= 0, +inf or NaN and 0 < y <= 1/4: random x over the whole positive range
+    including subnormals, the special arguments, and random exponents."""
+    x = np.concatenate([10.0 ** rng.uniform(-323, 308, 200000), rng.uniform(0.0, 2.0, 100000),
+                        [0.0, -0.0, np.inf, np.nan, 1.0, 5e-324, 2.2250738585072014e-308, 1.7976931348623157e308,
+                         0.9 ** -5, 1e30]])
+    for y in (np.full(x.size, 0.2), np.full(x.size, 0.25), rng.uniform(1e-3, 0.25, x.size)):
+        a = host_eval("pow_ctrl", x, y)
+        b = host_eval("pow", x, y)
+        nan = np.isnan(a) & np.isnan(b)
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.array_equal(a.view(np.uint64)[~nan], b.view(np.uint64)[~nan])
