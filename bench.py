@@ -359,12 +359,15 @@ def ring(args, mpgos, W, pyoracle):
     ns = args.nt or 4096
     U = 256
     spec = W.config4_ring(ns=ns, ups=U, periods=4)
-    kernels = {"mfma": 2, "lane": 1} if args.form == "auto" else {args.form: 2 if args.form == "mfma" else 1}
+    # (coupling kernel, units per lane of the dense-contraction kernel)
+    all_forms = {"mfma": (2, 4), "mfma2": (2, 2), "lane": (1, 4)}
+    kernels = all_forms if args.form == "auto" else {args.form: all_forms[args.form]}
     forms = {}
-    for name, k in kernels.items():
+    for name, (k, upl) in kernels.items():
         s = PBProblemSolver("ring_duffing", ns, U, spec["algorithm"])
         try:
             s.SolverOption(PBOption.CouplingKernel, k)
+            s.SolverOption(PBOption.MfmaUnitsPerLane, upl)
             times = []
             for i in range(args.warmup + args.steps):
                 W.load_pb(s, spec)
@@ -411,7 +414,7 @@ def ring(args, mpgos, W, pyoracle):
     line["forms"] = {k: {"kernel_ms": v["kernel_ms"], "system_steps_per_s": accepted / (v["kernel_ms"] * 1e-3)}
                      for k, v in forms.items()}
     if not args.no_adjoint:
-        line["adjoint"] = ring_adjoint(args, W, ns, U, kernels[best])
+        line["adjoint"] = ring_adjoint(args, W, ns, U, kernels[best][0])
     return line
 
 
@@ -542,7 +545,7 @@ def main():
     ap.add_argument("--nt", type=int, default=0, help="instances (systems for ring); 0 = the workload's size")
     ap.add_argument("--form", default="auto",
                     help="auto | plain | persistent | relaunch (per-thread; duffing_uneven also: single = one "
-                         "plain launch, relaunch = device-chained launches of --slice-steps); auto | mfma | lane (ring)")
+                         "plain launch, relaunch = device-chained launches of --slice-steps); auto | mfma | mfma2 | lane (ring; mfma2 = two units per lane)")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "f32k"])
     ap.add_argument("--algorithm", default="rk4", choices=["rk4", "rkck45"], help="duffing_adjoint only")
     ap.add_argument("--checkpoint-interval", type=int, default=0,

@@ -227,35 +227,47 @@ struct Unit {
 // two chains per tile (systems 0..3 and 4..7, independent, so one tile per
 // pass keeps two in flight).  Output (unit 4 ((lane / 4) % 4) + lane / 16 of
 // the tile, system lane % 4 (+ 4)) in one register per chain.
-template <int NC, int G = 8>
+//
+// The M tile values of a pass stream from L2 (a 256-unit matrix is 512 KiB) in
+// groups of G column blocks.  GLOBAL_M: M is in global memory, read with
+// global loads (a generic pointer gives flat loads, which also count as LDS
+// operations, so the wait for a group's B operands from LDS would wait for all
+// of its M loads too); otherwise M is in LDS.
+template <int NC, bool GLOBAL_M = true, int G = 8>
 __device__ __forceinline__ void mfma_contract44(const double* M, const double* cptB, double* coup, int UPS, int SPB)
 {
+    using GPtr = const __attribute__((address_space(1))) double*;
     const int l = threadIdx.x & 63;
     const int nw = blockDim.x >> 6;
     const int tiles = UPS >> 4;
     const int r16 = l & 15, k4 = l >> 4, sys = l & 3;
     const int drow = 4 * ((l >> 2) & 3) + (l >> 4);
-    const int blocks = UPS >> 2;
+    const int blocks = UPS >> 2;                        // a multiple of G (UPS is a multiple of 32)
+    const size_t colStride = (size_t)4 * UPS;
     // one 16-unit tile per pass; its two system halves are two independent chains
     for (int t = threadIdx.x >> 6; t < tiles; t += nw) {
 #pragma unroll
         for (int i = 0; i < NC; i++) {
             const double* Ai = M + (size_t)i * UPS * UPS + t * 16 + r16 + (size_t)k4 * UPS;
             const double* Bi = cptB + (size_t)i * UPS * 16 + k4 * 16 + sys;
+            auto ldA = [&](int blk) -> double {
+                if constexpr (GLOBAL_M) return *(GPtr)(Ai + (size_t)blk * colStride);
+                else return Ai[(size_t)blk * colStride];
+            };
+            auto step = [&](const double* a, int c0, double& acc0, double& acc1) {
+#pragma unroll
+                for (int d = 0; d < G; d++) {
+                    const double b0 = Bi[(c0 + d) * 64], b1 = Bi[(c0 + d) * 64 + 4];
+                    acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b0, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b1, acc1, 0, 0, 0);
+                }
+            };
             double acc0 = 0.0, acc1 = 0.0;              // systems 0..3, 4..7
             for (int c0 = 0; c0 < blocks; c0 += G) {
-                double a[G], b0[G], b1[G];
+                double a0[G];
 #pragma unroll
-                for (int d = 0; d < G; d++) {
-                    a[d] = Ai[(size_t)(c0 + d) * 4 * UPS];
-                    b0[d] = Bi[(c0 + d) * 64];
-                    b1[d] = Bi[(c0 + d) * 64 + 4];
-                }
-#pragma unroll
-                for (int d = 0; d < G; d++) {
-                    acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b0[d], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b1[d], acc1, 0, 0, 0);
-                }
+                for (int d = 0; d < G; d++) a0[d] = ldA(c0 + d);
+                step(a0, c0, acc0, acc1);
             }
             double* cp = coup + ((size_t)i * UPS + t * 16 + drow) * SPB;
             if (sys < SPB) cp[sys] = acc0;
@@ -421,7 +433,8 @@ __device__ void pb_body(const Args<Sys, P>& a)
                 }
             }
             __syncthreads();
-            mfma_contract44<NC>(M, cptBuf, coupBuf, UPS, SPB);
+            if (a.sharedMatrix) mfma_contract44<NC, false>(M, cptBuf, coupBuf, UPS, SPB);
+            else mfma_contract44<NC, true>(M, cptBuf, coupBuf, UPS, SPB);
             __syncthreads();
             if (run) {
 #pragma unroll
