@@ -462,9 +462,9 @@ __device__ __forceinline__ double rk_stage_time(int s, double t, double h)
     }
 }
 
-// LDS of the dense-contraction reverse pass: system scopes, the B operand
-// [coupling][unit][16] (columns 8..15 stay zero), the contraction results
-// [coupling][unit][system], the system-parameter gradient partials and two
+// LDS of the dense-contraction reverse pass: system scopes, the B operand and
+// the contraction results (rows of mfma_row(UPS) doubles per coupling and
+// block system, mfma_index), the system-parameter gradient partials and two
 // block-wide counters.
 struct RevMfmaLayout { size_t cptB, coup, red, blk, total; };
 
@@ -475,9 +475,9 @@ __host__ __device__ inline RevMfmaLayout reverse_mfma_layout(int UPS)
     RevMfmaLayout L;
     size_t o = ((size_t)SPB * sizeof(RevScope<Sys>) + 15) & ~(size_t)15;
     L.cptB = o;
-    o += (size_t)NCp * UPS * 16 * sizeof(double);
+    o += (size_t)NCp * SPB * mfma_row(UPS) * sizeof(double);
     L.coup = o;
-    o += (size_t)NCp * UPS * SPB * sizeof(double);
+    o += (size_t)NCp * SPB * mfma_row(UPS) * sizeof(double);
     L.red = o;
     o += (size_t)SPB * UPS * NSPp * sizeof(double);
     L.blk = o;
@@ -536,7 +536,7 @@ struct MfmaRevCtx {
         NU = (size_t)NS * UPS;
         gu0 = (unsigned)gsys * (unsigned)UPS + (unsigned)u0;
         my = &sys[lsys];
-        for (int i = tid; i < Pos<Sys::NC>::value * UPS * 16; i += blockDim.x) cptB[i] = 0.0;
+        for (int i = tid; i < Pos<Sys::NC>::value * kMfmaSystems * mfma_row(UPS); i += blockDim.x) cptB[i] = 0.0;
         if (tid == 0) blk[1] = 0;
         if (leader) {
             my->status = 0; my->k = -1; my->len = 0;
@@ -619,7 +619,7 @@ __device__ void pb_replay_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sy
                 Sys::template OdeFunction<P>(gsys, C.uid(p), F[p], Y[p], T, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
                                              my->sACC, my->sACCi, cp, CPF[p]);
 #pragma unroll
-                for (int i = 0; i < NC; i++) cptB[((size_t)i * UPS + C.uid(p)) * 16 + lsys] = cp[i];
+                for (int i = 0; i < NC; i++) cptB[mfma_index(i, lsys, C.uid(p), UPS)] = cp[i];
             }
         }
         lds_barrier();
@@ -630,7 +630,7 @@ __device__ void pb_replay_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sy
             for (int p = 0; p < PAIRS; p++)
 #pragma unroll
                 for (int i = 0; i < NC; i++) {
-                    v[p][i] = coup[((size_t)i * UPS + C.uid(p)) * SPB + lsys];
+                    v[p][i] = coup[mfma_index(i, lsys, C.uid(p), UPS)];
                     U::add(cIndex[i], my->strength[i] * CPF[p][i], v[p][i], F[p]);
                 }
         }
@@ -818,7 +818,7 @@ __device__ void pb_sweep_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys
 #pragma unroll
                         for (int c = 0; c < UD; c++)
                             if (c == cIndex[i]) g = kb[s][p][c];
-                        cptB[((size_t)i * UPS + C.uid(p)) * 16 + lsys] = my->strength[i] * cpf[i] * g;
+                        cptB[mfma_index(i, lsys, C.uid(p), UPS)] = my->strength[i] * cpf[i] * g;
                         kbCPF[p][i] = my->strength[i] * C.row(A.stg, base + UD + i)[C.at(p)] * g;
                     }
                 }
@@ -834,7 +834,7 @@ __device__ void pb_sweep_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys
 #pragma unroll
                     for (int c = 0; c < UD; c++) Y[c] = C.row(A.stg, base + c)[C.at(p)];
 #pragma unroll
-                    for (int i = 0; i < NC; i++) kbCPT[i] = coup[((size_t)i * UPS + C.uid(p)) * SPB + lsys];
+                    for (int i = 0; i < NC; i++) kbCPT[i] = coup[mfma_index(i, lsys, C.uid(p), UPS)];
                     Sys::template OdeVJP<P>(gsys, C.uid(p), kb[s][p], kbCPT, kbCPF[p], Y, Ts, uPAR[p], my->sPAR, gPAR, gPARi,
                                             uACC[p], uACCi[p], my->sACC, my->sACCi, gy, gU, gS);
 #pragma unroll

@@ -67,6 +67,18 @@ constexpr int kMaxBlock = 1024;
 constexpr int kMfmaSystems = 8;          // systems per workgroup of the dense-contraction form
 constexpr int kMfmaPairs = 4;            // default pairs per lane of the dense-contraction form (2 * UPS threads)
 
+// LDS of the dense-contraction form: the B operand (coupling terms) and the
+// contraction results, one row of UPS + 4 doubles per (coupling, block
+// system).  A wave's writes and reads of one system's consecutive units fall
+// in consecutive banks, and the B reads of a column block (4 columns x 4
+// systems, rows UPS + 4 apart: 8 banks apart per system) in 16 distinct bank
+// pairs.
+__host__ __device__ constexpr int mfma_row(int UPS) { return UPS + 4; }
+__host__ __device__ inline size_t mfma_index(int i, int s, int u, int UPS)
+{
+    return ((size_t)i * kMfmaSystems + s) * mfma_row(UPS) + u;
+}
+
 enum Coupling { kLaneCoupling = 0, kMfmaCoupling = 1 };
 
 template <class Sys, class P>
@@ -141,9 +153,9 @@ __host__ __device__ inline LdsLayout lds_layout(int cpl, int SPB, int UPS, int s
     o = (o + 15) & ~(size_t)15;
     L.cpt = o;
     if (cpl == kMfmaCoupling) {
-        o += (size_t)NCp * UPS * 16 * sizeof(P);
+        o += (size_t)NCp * kMfmaSystems * mfma_row(UPS) * sizeof(P);
         L.coup = o;
-        o += (size_t)NCp * UPS * SPB * sizeof(P);
+        o += (size_t)NCp * kMfmaSystems * mfma_row(UPS) * sizeof(P);
     } else {
         o += 2 * (size_t)SPB * UPS * NCp * sizeof(P);
         L.coup = o;
@@ -214,7 +226,7 @@ struct Unit {
 
 // Dense contraction of the coupling terms of the block's systems: for every
 // coupling i, unit u and system s < SPB (at most 8),
-//   coup[(i * UPS + u) * SPB + s] = sum_col M_i[u + col * UPS] * cptB[(i * UPS + col) * 16 + s]
+//   coup[mfma_index(i, s, u)] = sum_col M_i[u + col * UPS] * cptB[mfma_index(i, s, col)]
 // accumulated in column order, on v_mfma_f64_4x4x4f64 (called by every lane of
 // the workgroup: the instruction needs all 64 lanes of a wave; UPS is a
 // multiple of 16).  The instruction computes four independent
@@ -249,7 +261,8 @@ __device__ __forceinline__ void mfma_contract44(const double* M, const double* c
 #pragma unroll
         for (int i = 0; i < NC; i++) {
             const double* Ai = M + (size_t)i * UPS * UPS + t * 16 + r16 + (size_t)k4 * UPS;
-            const double* Bi = cptB + (size_t)i * UPS * 16 + k4 * 16 + sys;
+            const int RS = mfma_row(UPS);
+            const double* Bi = cptB + mfma_index(i, sys, k4, UPS);
             auto ldA = [&](int blk) -> double {
                 if constexpr (GLOBAL_M) return *(GPtr)(Ai + (size_t)blk * colStride);
                 else return Ai[(size_t)blk * colStride];
@@ -257,7 +270,7 @@ __device__ __forceinline__ void mfma_contract44(const double* M, const double* c
             auto step = [&](const double* a, int c0, double& acc0, double& acc1) {
 #pragma unroll
                 for (int d = 0; d < G; d++) {
-                    const double b0 = Bi[(c0 + d) * 64], b1 = Bi[(c0 + d) * 64 + 4];
+                    const double b0 = Bi[(c0 + d) * 4], b1 = Bi[(c0 + d) * 4 + 4 * RS];
                     acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b0, acc0, 0, 0, 0);
                     acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b1, acc1, 0, 0, 0);
                 }
@@ -269,9 +282,8 @@ __device__ __forceinline__ void mfma_contract44(const double* M, const double* c
                 for (int d = 0; d < G; d++) a0[d] = ldA(c0 + d);
                 step(a0, c0, acc0, acc1);
             }
-            double* cp = coup + ((size_t)i * UPS + t * 16 + drow) * SPB;
-            if (sys < SPB) cp[sys] = acc0;
-            if (sys + 4 < SPB) cp[sys + 4] = acc1;
+            if (sys < SPB) coup[mfma_index(i, sys, t * 16 + drow, UPS)] = acc0;
+            if (sys + 4 < SPB) coup[mfma_index(i, sys + 4, t * 16 + drow, UPS)] = acc1;
         }
     }
 }
@@ -330,8 +342,8 @@ __device__ void pb_body(const Args<Sys, P>& a)
         for (int i = tid; i < NC * a.cMatSize; i += blockDim.x) lm[i] = a.cMat[i];
         M = lm;
     }
-    if (CPL == kMfmaCoupling)   // B operand: columns of absent systems and 8..15 stay zero
-        for (int i = tid; i < NCp * UPS * 16; i += blockDim.x) cptBuf[i] = P(0);
+    if (CPL == kMfmaCoupling)   // B operand: rows of absent systems stay zero
+        for (int i = tid; i < NCp * kMfmaSystems * mfma_row(UPS); i += blockDim.x) cptBuf[i] = P(0);
     int cIndex[NCp];
 #pragma unroll
     for (int i = 0; i < NC; i++) cIndex[i] = a.cIndex[i];
@@ -429,7 +441,7 @@ __device__ void pb_body(const Args<Sys, P>& a)
                     Sys::template OdeFunction<P>(gsys, uid(p), F[p], Y[p], T, uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p],
                                                  my->sACC, my->sACCi, cp, CPF[p]);
 #pragma unroll
-                    for (int i = 0; i < NC; i++) cptBuf[((size_t)i * UPS + uid(p)) * 16 + lsys] = cp[i];
+                    for (int i = 0; i < NC; i++) cptBuf[mfma_index(i, lsys, uid(p), UPS)] = cp[i];
                 }
             }
             __syncthreads();
@@ -441,7 +453,7 @@ __device__ void pb_body(const Args<Sys, P>& a)
                 for (int p = 0; p < PAIRS; p++)
 #pragma unroll
                     for (int i = 0; i < NC; i++)
-                        U::add(cIndex[i], my->strength[i] * CPF[p][i], coupBuf[((size_t)i * UPS + uid(p)) * SPB + lsys], F[p]);
+                        U::add(cIndex[i], my->strength[i] * CPF[p][i], coupBuf[mfma_index(i, lsys, uid(p), UPS)], F[p]);
             }
         } else {
             P* cpt = cptBuf + (size_t)buf * cptStride;
