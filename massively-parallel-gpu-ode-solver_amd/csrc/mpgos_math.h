@@ -109,26 +109,32 @@ MPGOS_HD MPGOS_INLINE double reduce_pio2(double x, int& quadrant)
 }
 } // namespace detail
 
+// Non-finite arguments select x - x (NaN) at the end instead of returning
+// early: the function is then one block of arithmetic, which a compiler can
+// evaluate once for equal arguments (RK4's two mid-step stages share their
+// time, so a right-hand side cos(T) is computed once for both).
 MPGOS_HD MPGOS_INLINE double cos(double x)
 {
-    if (!isfinite(x)) return x - x;   // NaN for +-inf and NaN
+    const bool fin = isfinite(x);
     int q;
-    const double r = detail::reduce_pio2(x, q);
+    const double r = detail::reduce_pio2(fin ? x : 0.0, q);
     double c = detail::cos_kernel(r);
     double s = detail::sin_kernel(r);
     double v = (q & 1) ? s : c;
-    return (q == 1 || q == 2) ? -v : v;
+    v = (q == 1 || q == 2) ? -v : v;
+    return fin ? v : x - x;           // NaN for +-inf and NaN
 }
 
 MPGOS_HD MPGOS_INLINE double sin(double x)
 {
-    if (!isfinite(x)) return x - x;
+    const bool fin = isfinite(x);
     int q;
-    const double r = detail::reduce_pio2(x, q);
+    const double r = detail::reduce_pio2(fin ? x : 0.0, q);
     double c = detail::cos_kernel(r);
     double s = detail::sin_kernel(r);
     double v = (q & 1) ? c : s;
-    return (q >= 2) ? -v : v;
+    v = (q >= 2) ? -v : v;
+    return fin ? v : x - x;
 }
 
 // ---------------------------------------------------------------- exp

@@ -219,7 +219,7 @@ def test_device_math_matches_host_bitwise(gpu):
         L = gpu.lib()
         for name, fn in FN.items():
             arg = np.abs(x) if name in ("log", "pow", "pow_ctrl", "sqrt") else (np.clip(x, -700, 700) if name == "exp" else x)
-            if name in ("exp", "log", "pow", "pow_ctrl", "sqrt"):     # special arguments at the end
+            if name in ("cos", "sin", "exp", "log", "pow", "pow_ctrl", "sqrt"):     # special arguments at the end
                 arg = np.concatenate([arg[:-len(SPECIAL)], SPECIAL])
             arg = np.ascontiguousarray(arg)
             assert hip.hipMemcpy(bufs[0], arg.ctypes.data, nbytes, 1) == 0
