@@ -217,7 +217,7 @@ def per_thread(args, mpgos, W, pyoracle):
             pcfg = {"mode": 1, "block": 64, "round": 256, "waves_per_simd": 1, "order": 1, "solo": args.solo}
         forms["persistent"] = time_form(mpgos, spec, pcfg, args.steps, args.warmup, prec, sl)
     if args.form in ("auto", "relaunch") and not sl:
-        forms["relaunch"] = time_form(mpgos, spec, {"mode": 2, "block": 256, "relaunch": 100, "launches": 32},
+        forms["relaunch"] = time_form(mpgos, spec, {"mode": 2, "block": 256, "variant": 2, "relaunch": 200, "launches": 32},
                                       args.steps, args.warmup, prec)
     if args.form == "auto" and not args.no_tune and not sl:
         best_cfg = tune(mpgos, spec, prec)
