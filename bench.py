@@ -263,7 +263,7 @@ def per_thread(args, mpgos, W, pyoracle):
                          {"bound": "mfma", "engine": ENGINE_NOTE, "achieved": achieved,
                           "peak": FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": achieved / (FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS),
-                          "traffic": traffic_for(name), "flops_per_attempted_step": fpa, "kernel_ms": res["kernel_ms"]},
+                          "traffic": traffic_for(name) if prec == 0 else None, "flops_per_attempted_step": fpa, "kernel_ms": res["kernel_ms"]},
                          cpu, parity)
     line["forms"] = {k: {"config": v["config"], "wall_ms_per_step": v["wall_ms_per_step"], "kernel_ms": v["kernel_ms"],
                          "instance_steps_per_s": accepted / (v["wall_ms_per_step"] * 1e-3)} for k, v in forms.items()}
