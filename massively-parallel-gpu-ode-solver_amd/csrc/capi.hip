@@ -20,6 +20,7 @@ static const Registration kSystems[] = {
     {"duffing_eps", make_pt_duffing_eps},
     {"duffing_dw", make_pt_duffing_dw},
     {"relief_valve", make_pt_relief_valve},
+    {"keller_miksis", make_pt_keller_miksis},
     {"lorenz", make_pt_lorenz},
     {"affine2", make_pt_affine2},
 };

@@ -25,6 +25,7 @@ SYSTEM_DIMS = {
     "duffing_eps": (2, 3, 0, 0, 1, 0, 0),
     "duffing_dw":  (2, 1, 1, 0, 2, 3, 1),
     "relief_valve": (3, 1, 4, 0, 2, 3, 2),
+    "keller_miksis": (2, 21, 0, 0, 1, 4, 1),
     "lorenz":      (3, 1, 0, 0, 0, 0, 0),
     "affine2":     (2, 6, 1, 0, 1, 0, 1),
 }
@@ -102,6 +103,53 @@ def tutorial_relief_valve(nt=256):
     s["rtol"] = 1e-10
     s["atol"] = 1e-10
     s["edir"] = np.array([-1, -1])
+    return s
+
+
+def keller_miksis_parameters(pa1, f1, pa2, f2, phase=0.0, re_um=10.0, pinf_bar=1.0, kappa=1.4):
+    """The 21 control parameters of one Keller-Miksis instance from pressure
+    amplitudes [bar] and frequencies [kHz], as the reference's quasi-periodic
+    forcing tutorial forms them (QuasiperiodicForcing.cu:248-322; water at
+    room temperature)."""
+    pv, rho, st, vis, cl = 3.166775638952003e+03, 9.970639504998557e+02, 0.071977583160056, \
+        8.902125058209557e-04, 1.497251785455527e+03
+    pinf, PA1, PA2, RE = pinf_bar * 1.0e5, pa1 * 1.0e5, pa2 * 1.0e5, re_um / 1.0e6
+    w1, w2 = 2.0 * PI * (f1 * 1000), 2.0 * PI * (f2 * 1000)
+    g = (2.0 * PI / RE / w1) ** 2.0
+    return [(2.0 * st / RE + pinf - pv) * g / rho,
+            (1.0 - 3.0 * kappa) * (2 * st / RE + pinf - pv) * (2.0 * PI / RE / w1) / cl / rho,
+            (pinf - pv) * g / rho,
+            (2.0 * st / RE / rho) * g,
+            (4.0 * vis / rho / RE ** 2.0) * (2.0 * PI / w1),
+            PA1 * g / rho,
+            PA2 * g / rho,
+            (RE * w1 * PA1 / rho / cl) * g,
+            (RE * w2 * PA2 / rho / cl) * g,
+            RE * w1 / (2.0 * PI) / cl,
+            3.0 * kappa,
+            f2 / f1,
+            phase,
+            2.0 * PI / w1, RE, pa1, f1, pa2, f2, phase, re_um]
+
+
+def tutorial_keller_miksis(n1=16, n2=16, pa1=0.5, pa2=0.7):
+    """The reference's quasi-periodic forcing tutorial setup
+    (Tutorials_SingleSystem_PerThread/T4_QuasiperiodicForcing/QuasiperiodicForcing.cu:37-72, 248-322)
+    on an n1 x n2 grid of the two frequencies, logspace(20, 1000) kHz each:
+    (y1, y2) = (1, 0), time domain [0, 1e10] (a Solve ends at the next local
+    maximum of the radius), RKCK45 at rtol = atol = 1e-10, event direction -1;
+    repeated Solves continue from the last state."""
+    nt = n1 * n2
+    s = new_spec("keller_miksis", Algorithms.RKCK45, nt)
+    f1 = np.logspace(np.log10(20.0), np.log10(1000.0), n1)
+    f2 = np.logspace(np.log10(20.0), np.log10(1000.0), n2)
+    for i in range(nt):
+        s["cPAR"][:, i] = keller_miksis_parameters(pa1, f1[i % n1], pa2, f2[i // n1])
+    s["TD"][1] = 1.0e10
+    s["X"][0] = 1.0
+    s["rtol"] = 1e-10
+    s["atol"] = 1e-10
+    s["edir"] = np.array([-1])
     return s
 
 

@@ -94,6 +94,9 @@ def all_scenarios():
     out.append(("tutorial_duffing_doublewell", "duffing_dw", dw))
     # reference impact tutorial: state jump at impact, user termination (first maximum, or resting valve)
     out.append(("tutorial_relief_valve", "relief_valve", W.tutorial_relief_valve(nt=256)))
+    # reference quasi-periodic forcing tutorial: Keller-Miksis bubble, pow and two
+    # forcing frequencies in the right-hand side, termination at the first maximum
+    out.append(("tutorial_keller_miksis", "keller_miksis", W.tutorial_keller_miksis(16, 16)))
     # ragged: only part of the instances active
     lz = W.config1_lorenz(nt=1000)
     lz["n_active"] = 777

@@ -184,6 +184,22 @@ def test_repeated_solves_continue_the_integration(gpu, oracle):
         cur["X"], cur["T"], cur["ACC"] = ref["X"], ref["T"], ref["ACC"]
 
 
+def test_keller_miksis_continuation_solves(gpu, oracle):
+    # the quasi-periodic forcing tutorial chains Solve() calls, each from the
+    # last state to the next maximum of the radius (QuasiperiodicForcing.cu:134-160)
+    spec = W.tutorial_keller_miksis(16, 16)
+    for mode in (0, 1):
+        print(f"keller-miksis continuation mode={mode}", flush=True)
+        outs = run_gpu(spec, kernel_mode=mode, solves=3)
+        cur = dict(spec)
+        for k, out in enumerate(outs):
+            ref = oracle.run("keller_miksis", cur)
+            assert_bitwise(out, ref, keys=("X", "T", "ACC", "ACCi", "nacc", "nrej", "status"), label=f"solve {k} mode {mode}")
+            cur = dict(cur)
+            cur["X"], cur["T"] = ref["X"], ref["T"]
+        assert (outs[-1]["ACCi"][0] == 1).all()                # every Solve ended at the next maximum
+
+
 def test_relief_valve_continuation_solves(gpu, oracle):
     # the impact tutorial chains Solve() calls, each from the last state to the
     # next maximum (ImpactDynamics.cu:90-117); state jumps and user terminations

@@ -335,3 +335,45 @@ def test_relief_valve_impacts_and_user_termination_match_scipy(oracle):
         assert np.allclose(out["X"][[0, 2], i], y_end[[0, 2]], rtol=1e-5, atol=1e-6), (i, out["X"][:, i], y_end)
         assert abs(out["X"][1, i]) < 1e-6                              # v within the event tolerance
     assert bounced >= 2
+
+
+def _keller_miksis_rhs(c):
+    def f(t, y):
+        rx1 = 1.0 / y[0]
+        p = rx1 ** c[10]
+        s1, c1 = math.sin(2 * W.PI * t), math.cos(2 * W.PI * t)
+        a2 = 2.0 * c[11] * W.PI * t + c[12]
+        s2, c2 = math.sin(a2), math.cos(a2)
+        N = ((c[0] + c[1] * y[1]) * p - c[2] * (1.0 + c[9] * y[1]) - c[3] * rx1 - c[4] * y[1] * rx1
+             - 1.5 * (1.0 - c[9] * y[1] / 3.0) * y[1] ** 2 - (c[5] * s1 + c[6] * s2) * (1.0 + c[9] * y[1])
+             - y[0] * (c[7] * c1 + c[8] * c2))
+        D = y[0] - c[9] * y[0] * y[1] + c[4] * c[9]
+        return [y[1], N / D]
+    return f
+
+
+def test_keller_miksis_first_maximum_matches_scipy(oracle):
+    """Quasi-periodic forcing tutorial (Keller-Miksis bubble, two forcing
+    frequencies, user termination at the first maximum of the radius): the
+    oracle's end time and radius agree with an event-driven SciPy integration
+    (DOP853 at 1e-12, terminal event y2 = 0 falling) to the event tolerance's
+    effect, and the minimum-radius accessory with the SciPy minimum before it."""
+    spec = W.tutorial_keller_miksis(4, 4)
+    out = oracle.run("keller_miksis", spec)
+    for i in range(spec["nt"]):
+        c = spec["cPAR"][:, i]
+        ev = lambda t, y: y[1]
+        ev.direction = -1
+        sol = solve_ivp(_keller_miksis_rhs(c), (0.0, 2.0 * out["T"][i] + 1.0), [1.0, 0.0], method="DOP853",
+                        rtol=1e-12, atol=1e-12, events=ev, dense_output=True)
+        # y2 starts at 0: the solver's event test needs y2 above the tolerance
+        # first, so the start is not an event; the first maximum after it is
+        k = int(np.argmax(sol.t_events[0] > 1e-9))
+        t_end = sol.t_events[0][k]
+        y_end = sol.y_events[0][k]
+        assert out["ACCi"][0, i] == 1                                   # ended by the first maximum
+        assert abs(out["T"][i] - t_end) < 1e-5 * max(1.0, t_end), (i, out["T"][i], t_end)
+        assert abs(out["X"][0, i] - y_end[0]) < 1e-6 * abs(y_end[0]), (i, out["X"][0, i], y_end[0])
+        assert abs(out["X"][1, i]) < 1e-5
+        tt = np.linspace(0.0, t_end, 20001)
+        assert out["ACC"][2, i] == pytest.approx(sol.sol(tt)[0].min(), rel=1e-4)
