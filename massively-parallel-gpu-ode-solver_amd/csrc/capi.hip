@@ -21,6 +21,7 @@ static const Registration kSystems[] = {
     {"duffing_dw", make_pt_duffing_dw},
     {"relief_valve", make_pt_relief_valve},
     {"keller_miksis", make_pt_keller_miksis},
+    {"keller_miksis_max", make_pt_keller_miksis_max},
     {"lorenz", make_pt_lorenz},
     {"affine2", make_pt_affine2},
 };

@@ -52,6 +52,7 @@ SolverBase* make_pt_duffing_eps(int precision);
 SolverBase* make_pt_duffing_dw(int precision);
 SolverBase* make_pt_relief_valve(int precision);
 SolverBase* make_pt_keller_miksis(int precision);
+SolverBase* make_pt_keller_miksis_max(int precision);
 SolverBase* make_pt_lorenz(int precision);
 SolverBase* make_pt_affine2(int precision);
 

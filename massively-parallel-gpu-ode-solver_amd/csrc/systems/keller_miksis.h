@@ -79,3 +79,40 @@ struct KellerMiksis : NoCallbacks {
 
 } // namespace systems
 } // namespace mpgos
+
+namespace mpgos {
+namespace systems {
+
+// The same bubble under one forcing frequency, as the reference's large time
+// scale differences tutorial runs it
+// (Tutorials_SingleSystem_PerThread/T5_LargeTimeScaleDifferences/KellerMiksis_SystemDefinition.cuh:7-94):
+// 13 control parameters (c6 = c8 = c11 = 0), no events, every Solve restarts
+// the time at TimeDomain[0] (one forcing period per Solve) and ACC[0] keeps
+// the largest radius of the Solve.
+struct KellerMiksisMax : NoCallbacks {
+    static constexpr const char* name = "keller_miksis_max";
+    static constexpr int SD = 2, NCP = 13, NSP = 0, NISP = 0, NE = 0, NA = 1, NIA = 0;
+    static constexpr bool HAS_VJP = false;
+
+    template <class P>
+    MPGOS_HD static void OdeFunction(int tid, int NT, P* F, P* X, P T, P* c, P* s, int* si, P* a, int* ai)
+    {
+        KellerMiksis::OdeFunction<P>(tid, NT, F, X, T, c, s, si, a, ai);
+    }
+
+    template <class P>
+    MPGOS_HD static void ActionAfterSuccessfulTimeStep(int, int, int&, P&, P&, P*, P* X, P*, P*, int*, P* ACC, int*)
+    {
+        if (X[0] > ACC[0]) ACC[0] = X[0];
+    }
+
+    template <class P>
+    MPGOS_HD static void Initialization(int, int, int&, P& T, P&, P* TD, P* X, P*, P*, int*, P* ACC, int*)
+    {
+        T = TD[0];
+        ACC[0] = X[0];
+    }
+};
+
+} // namespace systems
+} // namespace mpgos

@@ -26,6 +26,7 @@ SYSTEM_DIMS = {
     "duffing_dw":  (2, 1, 1, 0, 2, 3, 1),
     "relief_valve": (3, 1, 4, 0, 2, 3, 2),
     "keller_miksis": (2, 21, 0, 0, 1, 4, 1),
+    "keller_miksis_max": (2, 13, 0, 0, 0, 1, 0),
     "lorenz":      (3, 1, 0, 0, 0, 0, 0),
     "affine2":     (2, 6, 1, 0, 1, 0, 1),
 }
@@ -150,6 +151,24 @@ def tutorial_keller_miksis(n1=16, n2=16, pa1=0.5, pa2=0.7):
     s["rtol"] = 1e-10
     s["atol"] = 1e-10
     s["edir"] = np.array([-1])
+    return s
+
+
+def tutorial_keller_miksis_max(nt=64):
+    """The reference's large time scale differences tutorial setup
+    (Tutorials_SingleSystem_PerThread/T5_LargeTimeScaleDifferences/KellerMiksis.cu:36-58, 151-210):
+    one forcing of 1.5 bar at logspace(20, 1000) kHz per instance, (y1, y2) =
+    (1, 0), time domain [0, 1] (one forcing period per Solve, the time
+    restarting at 0), RKCK45 at rtol = atol = 1e-10; ACC[0] = largest radius
+    of a Solve."""
+    s = new_spec("keller_miksis_max", Algorithms.RKCK45, nt)
+    f1 = np.logspace(np.log10(20.0), np.log10(1000.0), nt)
+    for i in range(nt):
+        s["cPAR"][:, i] = keller_miksis_parameters(1.5, f1[i], 0.0, 0.0)[:13]
+    s["TD"][1] = 1.0
+    s["X"][0] = 1.0
+    s["rtol"] = 1e-10
+    s["atol"] = 1e-10
     return s
 
 

@@ -106,6 +106,7 @@ static const Entry kSystems[] = {
     {"duffing_dw", run_all<mpgos::systems::DuffingDW, double>, run_all<mpgos::systems::DuffingDW, flops::F>, trace_entry<mpgos::systems::DuffingDW>, sensitivity_all<mpgos::systems::DuffingDW>},
     {"relief_valve", run_all<mpgos::systems::ReliefValve, double>, run_all<mpgos::systems::ReliefValve, flops::F>, trace_entry<mpgos::systems::ReliefValve>, sensitivity_all<mpgos::systems::ReliefValve>},
     {"keller_miksis", run_all<mpgos::systems::KellerMiksis, double>, run_all<mpgos::systems::KellerMiksis, flops::F>, trace_entry<mpgos::systems::KellerMiksis>, sensitivity_all<mpgos::systems::KellerMiksis>},
+    {"keller_miksis_max", run_all<mpgos::systems::KellerMiksisMax, double>, run_all<mpgos::systems::KellerMiksisMax, flops::F>, trace_entry<mpgos::systems::KellerMiksisMax>, sensitivity_all<mpgos::systems::KellerMiksisMax>},
     {"lorenz", run_all<mpgos::systems::Lorenz, double>, run_all<mpgos::systems::Lorenz, flops::F>, trace_entry<mpgos::systems::Lorenz>, sensitivity_all<mpgos::systems::Lorenz>},
     {"affine2", run_all<mpgos::systems::Affine2, double>, run_all<mpgos::systems::Affine2, flops::F>, trace_entry<mpgos::systems::Affine2>, sensitivity_all<mpgos::systems::Affine2>},
 };

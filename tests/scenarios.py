@@ -97,6 +97,8 @@ def all_scenarios():
     # reference quasi-periodic forcing tutorial: Keller-Miksis bubble, pow and two
     # forcing frequencies in the right-hand side, termination at the first maximum
     out.append(("tutorial_keller_miksis", "keller_miksis", W.tutorial_keller_miksis(16, 16)))
+    # reference large time scale differences tutorial: one period per Solve, time reset, max accessory
+    out.append(("tutorial_keller_miksis_max", "keller_miksis_max", W.tutorial_keller_miksis_max(64)))
     # ragged: only part of the instances active
     lz = W.config1_lorenz(nt=1000)
     lz["n_active"] = 777

@@ -200,6 +200,21 @@ def test_keller_miksis_continuation_solves(gpu, oracle):
         assert (outs[-1]["ACCi"][0] == 1).all()                # every Solve ended at the next maximum
 
 
+def test_keller_miksis_max_period_solves(gpu, oracle):
+    # the large time scale differences tutorial runs one forcing period per
+    # Solve, the time restarting at 0 (KellerMiksis.cu:63-90)
+    spec = W.tutorial_keller_miksis_max(64)
+    print("keller-miksis periods", flush=True)
+    outs = run_gpu(spec, solves=4)
+    cur = dict(spec)
+    for k, out in enumerate(outs):
+        ref = oracle.run("keller_miksis_max", cur)
+        assert_bitwise(out, ref, keys=("X", "T", "ACC", "nacc", "nrej", "status"), label=f"period {k}")
+        cur = dict(cur)
+        cur["X"] = ref["X"]
+    assert (outs[-1]["ACC"][0] >= outs[-1]["X"][0]).all()
+
+
 def test_relief_valve_continuation_solves(gpu, oracle):
     # the impact tutorial chains Solve() calls, each from the last state to the
     # next maximum (ImpactDynamics.cu:90-117); state jumps and user terminations

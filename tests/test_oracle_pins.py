@@ -377,3 +377,18 @@ def test_keller_miksis_first_maximum_matches_scipy(oracle):
         assert abs(out["X"][1, i]) < 1e-5
         tt = np.linspace(0.0, t_end, 20001)
         assert out["ACC"][2, i] == pytest.approx(sol.sol(tt)[0].min(), rel=1e-4)
+
+
+def test_keller_miksis_one_period_matches_scipy(oracle):
+    """Large time scale differences tutorial (single forcing, one period per
+    Solve): the oracle's state after one period and its largest-radius
+    accessory agree with SciPy (DOP853 at 1e-12) to 1e-7 relative."""
+    spec = W.tutorial_keller_miksis_max(8)
+    out = oracle.run("keller_miksis_max", spec)
+    for i in range(spec["nt"]):
+        c = list(spec["cPAR"][:, i]) + [0.0] * 8
+        sol = solve_ivp(_keller_miksis_rhs(c), (0.0, 1.0), [1.0, 0.0], method="DOP853", rtol=1e-12, atol=1e-12,
+                        dense_output=True)
+        assert np.allclose(out["X"][:, i], sol.y[:, -1], rtol=1e-7, atol=1e-9), (i, out["X"][:, i], sol.y[:, -1])
+        ymax = sol.sol(np.linspace(0.0, 1.0, 200001))[0].max()
+        assert out["ACC"][0, i] == pytest.approx(ymax, rel=1e-5)
