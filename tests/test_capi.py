@@ -37,7 +37,8 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_registered_systems():
-    assert set(mpgos.list_systems()) >= {"duffing", "duffing_q", "duffing_eps", "duffing_dw", "lorenz", "affine2"}
+    assert set(mpgos.list_systems()) >= {"duffing", "duffing_q", "duffing_eps", "duffing_dw", "lorenz", "affine2",
+                                         "relief_valve", "keller_miksis", "keller_miksis_max"}
 
 
 def test_errors_are_return_codes():
