@@ -2,7 +2,11 @@
 // Tutorials_SingleSystem_PerThread/T2_Lorenz/Lorenz.cu), compiled unchanged
 // against the drop-in header.  Its callbacks forward to the registered Lorenz
 // system so the GPU test can compare the result with the oracle bit for bit.
-//   usage: lorenz_dropin <inputs.bin> <outputs.bin> <NT>
+// It also keeps a dense output (32 points, at least 0.5 time units apart) and
+// writes the reference's Print files (ActualState, TimeDomain, ControlParameters,
+// ActualTime, and DenseOutput for instances 0 and 1234) into the working
+// directory, so the test can compare their text with the oracle's values.
+//   usage: lorenz_dropin <inputs.bin> <outputs.bin>
 // inputs.bin : rho[NT], x[3][NT] (float64);  outputs.bin: x[3][NT], t[NT], accepted[NT] (float64)
 #include <cstdio>
 #include <vector>
@@ -45,7 +49,7 @@ const int NISP = 0;
 const int NE = 0;
 const int NA = 0;
 const int NIA = 0;
-const int NDO = 0;
+const int NDO = 32;
 
 int main(int argc, char** argv)
 {
@@ -57,6 +61,8 @@ int main(int argc, char** argv)
 
     ProblemSolver<NT, SD, NCP, NSP, NISP, NE, NA, NIA, NDO, SOLVER, PRECISION> ScanLorenz(SelectDeviceByClosestRevision(9, 5));
     ScanLorenz.SolverOption(ThreadsPerBlock, 64);
+    ScanLorenz.SolverOption(DenseOutputMinimumTimeStep, 0.5);
+    ScanLorenz.SolverOption(DenseOutputSaveFrequency, 1);
     for (int i = 0; i < SD; i++) {
         ScanLorenz.SolverOption(RelativeTolerance, i, 1e-10);
         ScanLorenz.SolverOption(AbsoluteTolerance, i, 1e-10);
@@ -67,12 +73,21 @@ int main(int argc, char** argv)
         ScanLorenz.SetHost(tid, ActualTime, 0.0);
         ScanLorenz.SetHost(tid, ControlParameters, 0, in[tid]);
         for (int c = 0; c < SD; c++) ScanLorenz.SetHost(tid, ActualState, c, in[NT + c * NT + tid]);
+        ScanLorenz.SetHost(tid, DenseIndex, 0);
     }
     ScanLorenz.SynchroniseFromHostToDevice(All);
     ScanLorenz.Solve();
     ScanLorenz.SynchroniseFromDeviceToHost(All);
+    ScanLorenz.SynchroniseFromDeviceToHost(DenseOutput);
     ScanLorenz.InsertSynchronisationPoint();
     ScanLorenz.SynchroniseSolver();
+
+    ScanLorenz.Print(ActualState);
+    ScanLorenz.Print(TimeDomain);
+    ScanLorenz.Print(ControlParameters);
+    ScanLorenz.Print(ActualTime);
+    ScanLorenz.Print(DenseOutput, 0);
+    ScanLorenz.Print(DenseOutput, 1234);
 
     std::vector<double> out(5 * NT);
     for (int tid = 0; tid < NT; tid++) {

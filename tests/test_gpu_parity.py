@@ -288,18 +288,52 @@ def test_fp32_modes_error_against_fp64_oracle(gpu, oracle, precision):
     assert rel < 1e-3, rel
 
 
+def _fmt(v):
+    """One value as the reference's Print writes it: width 18, precision 10,
+    scientific (SingleSystem_PerThread_Interface.cuh:1091-1099)."""
+    return f"{v:18.10e}"
+
+
 def test_dropin_cpp_driver_matches_oracle(gpu, oracle, tmp_path):
+    """A driver written against the reference's per-thread interface, with a
+    dense output of 32 points at least 0.5 apart: end states bit for bit, and
+    the text of its Print files (general and DenseOutput for two instances)
+    character for character equal to the oracle's values written in the
+    reference's format."""
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(__file__), "cpp", "lorenz_dropin")
     assert os.path.exists(exe), "build() compiles tests/cpp/lorenz_dropin"
-    spec = W.config1_lorenz(nt=4096)
+    nt = 4096
+    spec = W.config1_lorenz(nt=nt)
     inp = np.concatenate([spec["cPAR"][0], spec["X"].ravel()]).astype(np.float64)
     (tmp_path / "in.bin").write_bytes(inp.tobytes())
     r = subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], capture_output=True, text=True,
                        timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stderr
-    out = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.float64).reshape(5, 4096)
-    ref = oracle.run("lorenz", spec)
+    out = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.float64).reshape(5, nt)
+    dspec = dict(spec)
+    dspec["ndo"] = 32
+    dspec["options"] = dict(spec["options"], do_min_dt=0.5, do_freq=1)
+    ref = oracle.run("lorenz", dspec)
     assert np.array_equal(out[:3].view(np.uint64), ref["X"].view(np.uint64))
     assert np.array_equal(out[3], ref["T"]) and np.array_equal(out[4].astype(np.int64), ref["nacc"])
+
+    def table(name):
+        return (tmp_path / name).read_text().split("\n")[:-1]
+
+    rows = table("ActualStateInSolverObject.txt")
+    assert rows == [",".join(_fmt(ref["X"][c, i]) for c in range(3)) for i in range(nt)]
+    assert table("TimeDomainInSolverObject.txt") == [f"{_fmt(0.0)},{_fmt(10.0)}"] * nt
+    assert table("ControlParametersInSolverObject.txt") == [_fmt(v) for v in spec["cPAR"][0]]
+    assert table("ActualTimeInSolverObject.txt") == [_fmt(v) for v in ref["T"]]
+    for tid in (0, 1234):
+        n = int(ref["DOidx"][tid])
+        assert 15 <= n <= 32                      # ~one point per 0.5 time units over [0, 10], plus the end
+        expect = ["ControlParameters:", _fmt(spec["cPAR"][0, tid]), "SharedParameters:", "",
+                  "IntegerSharedParameters:", "", "Accessories:", "", "IntegerAccessories:", "", "",
+                  "Time series:"]
+        expect += [_fmt(ref["DOt"][k, tid]) + "," + ",".join(_fmt(ref["DOx"][k, c, tid]) for c in range(3))
+                   for k in range(n)]
+        assert table(f"DenseOutput_{tid}.txt") == expect, tid
+        assert float(expect[-1].split(",")[0]) == ref["T"][tid] == 10.0
