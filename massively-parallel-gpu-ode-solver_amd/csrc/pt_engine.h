@@ -595,7 +595,7 @@ public:
             b.listLen = nActive;
             b.listLenDev = s == 0 ? nullptr : relaunchCounts + (s - 1);
             b.outCount = relaunchCounts + s;
-            b.slice = s < S ? relaunchSteps : 0x7fffffff;
+            b.slice = s < S ? relaunchSteps : 0;
             hipLaunchKernelGGL(fn, dim3(grid), dim3(block()), dyn_lds(), strm, b);
         }
         MPGOS_HIP_TRY(hipGetLastError());
