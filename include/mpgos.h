@@ -212,6 +212,24 @@ int mpgos_pt_solve(mpgos_solver* h);
  * longest instance runs for minutes: the caller regains control between
  * slices (progress, cancellation).  Synchronous; not with GraphCapture. */
 int mpgos_pt_solve_slice(mpgos_solver* h, long long max_attempted_steps, int* unfinished);
+/* New, no counterpart in the reference: Solve() for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Runs the relaunch form (MPGOS_KERNEL_RELAUNCH,
+ * with the handle's ThreadsPerBlock, LaunchVariant, RelaunchSteps and
+ * RelaunchLaunches, whatever KernelMode says) over the handle's own buffers,
+ * with its lane records, instance lists and counts in the library's static
+ * scratch memory of MPGOS_NOALLOC_SCRATCH_BYTES per GPU instead of memory the
+ * handle allocates; results are bit-identical to mpgos_pt_solve.  Calls no
+ * hipMalloc/hipFree and captures no graph.  Needs
+ * NT * (record bytes + 12) + 4 * (RelaunchLaunches + 1) bytes, the record
+ * being the instance's lane state rounded up to 16 bytes (160 for Lorenz in
+ * FP64, i.e. up to about 390000 instances); larger handles get MPGOS_E_ARG.
+ * Every call on one GPU uses that GPU's single scratch block: calls that
+ * overlap on one GPU (two handles whose streams run concurrently) race, so
+ * order them on one stream or synchronise between them.  Asynchronous on the
+ * handle's stream, like mpgos_pt_solve. */
+#define MPGOS_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int mpgos_pt_solve_noalloc(mpgos_solver* h);
 
 /* ProblemSolver::SynchroniseDevice / InsertSynchronisationPoint / SynchroniseSolver
  *   (SingleSystem_PerThread_Interface.cuh:1288-1312). */

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "mpgos.h"
@@ -52,6 +54,44 @@ __global__ void math_eval_kernel(int fn, const double* x, const double* y, doubl
     }
 }
 
+// Scratch memory of mpgos_pt_solve_noalloc: the relaunch form's lane records,
+// phases, instance lists and counts.  One copy per GPU, held by the loaded code
+// object until the process ends.  Its host-side name is not a device address:
+// host code uses only the address hipGetSymbolAddress returned for the device.
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mutex;
+std::map<int, void*> g_scratch_addr;        // device -> address of `scratch` on that device
+std::map<int, std::string> g_scratch_error; // device -> why the lookup failed
+
+// Once per device, with that device current (called when a solver is created).
+void scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_addr.count(device) || g_scratch_error.count(device)) return;
+    hipError_t e = hipSetDevice(device);
+    void* p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        g_scratch_error[device] = std::string("scratch memory lookup failed: ") + hipGetErrorString(e);
+        return;
+    }
+    g_scratch_addr[device] = p;
+}
+
+void* scratch_address(int device, std::string& why)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) return it->second;
+    auto er = g_scratch_error.find(device);
+    why = er != g_scratch_error.end() ? er->second : "scratch memory not set up for this device";
+    return nullptr;
+}
+} // namespace
+
 } // namespace mpgos
 
 struct mpgos_solver {
@@ -87,6 +127,7 @@ int mpgos_pt_create(const char* system, int algorithm, int precision, int nt, in
         if (!s) { set_error("mpgos_pt_create: unsupported precision"); return MPGOS_E_SYSTEM; }
         int rc = s->init(algorithm, nt, ndo, device);
         if (rc != MPGOS_OK) { delete s; return rc; }
+        mpgos::scratch_setup(device);
         mpgos_solver* h = new mpgos_solver;
         h->pt = s;
         *out = h;
@@ -155,6 +196,14 @@ int mpgos_pt_solve_slice(mpgos_solver* h, long long max_attempted_steps, int* un
     MPGOS_CHECK_HANDLE(h);
     if (!unfinished) { set_error("mpgos_pt_solve_slice: null pointer"); return MPGOS_E_ARG; }
     return h->pt->solve_slice(max_attempted_steps, unfinished);
+}
+int mpgos_pt_solve_noalloc(mpgos_solver* h)
+{
+    MPGOS_CHECK_HANDLE(h);
+    std::string why;
+    void* p = mpgos::scratch_address(h->pt->device_index(), why);
+    if (!p) { set_error("mpgos_pt_solve_noalloc: " + why); return MPGOS_E_STATE; }
+    return h->pt->solve_noalloc(p, mpgos::kNoallocScratchBytes);
 }
 int mpgos_pt_synchronise_device(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->synchronise_device(); }
 int mpgos_pt_insert_synchronisation_point(mpgos_solver* h) { MPGOS_CHECK_HANDLE(h); return h->pt->insert_sync_point(); }

@@ -84,6 +84,8 @@ public:
     virtual int last_solve_ms(double* ms) const = 0;
     virtual void* stream() = 0;
     virtual int solve_slice(long long max_attempted_steps, int* unfinished) = 0;
+    virtual int solve_noalloc(void* scratch, size_t bytes) = 0;
+    virtual int device_index() const = 0;
     virtual int adjoint_record(int K, int capacity) = 0;
     virtual int adjoint_reverse() = 0;
 };
@@ -578,27 +580,79 @@ public:
         return MPGOS_OK;
     }
 
+    // Device memory of the relaunch form: the lane records of suspended
+    // instances, their phases, the two instance lists of consecutive launches
+    // and the per-launch suspended counts.
+    struct RelaunchBuffers {
+        uint32_t* saved; int* phase; int* listA; int* listB; unsigned* counts;
+    };
+    // Bytes of RelaunchBuffers carved from one block (records first, 16-byte
+    // multiples; the rest 4-byte words).
+    size_t relaunch_bytes() const
+    {
+        return lane_words() * 4 * (size_t)NT + 3 * 4 * (size_t)NT + 4 * (size_t)(relaunchLaunches + 1);
+    }
+    RelaunchBuffers carve_relaunch(void* base) const
+    {
+        RelaunchBuffers r;
+        unsigned char* p = static_cast<unsigned char*>(base);
+        r.saved = reinterpret_cast<uint32_t*>(p); p += lane_words() * 4 * (size_t)NT;
+        r.phase = reinterpret_cast<int*>(p); p += 4 * (size_t)NT;
+        r.listA = reinterpret_cast<int*>(p); p += 4 * (size_t)NT;
+        r.listB = reinterpret_cast<int*>(p); p += 4 * (size_t)NT;
+        r.counts = reinterpret_cast<unsigned*>(p);
+        return r;
+    }
+
     // Relaunch form: relaunchLaunches launches of relaunchSteps attempted steps
     // per instance, each working on the previous launch's suspended instances
     // packed into the lowest lanes (list length read on the device), then one
     // launch without a budget; queued together, no host round trip.
-    int enqueue_relaunch(const Args<Sys, P, CP>& a, KernelFn fn, int grid)
+    int enqueue_relaunch(const Args<Sys, P, CP>& a, KernelFn fn, int grid, int B, size_t lds, const RelaunchBuffers& r)
     {
         const int S = relaunchLaunches;
         hipLaunchKernelGGL(reset_relaunch_kernel<0>, dim3((unsigned)((NT > S + 1 ? NT : S + 1) + 255) / 256), dim3(256), 0, strm,
-                           phase, NT, relaunchCounts, S + 1);
+                           r.phase, NT, r.counts, S + 1);
         for (int s = 0; s <= S; s++) {
             Args<Sys, P, CP> b = a;
-            b.saved = saved; b.phase = phase;
-            b.list = s == 0 ? a.list : (s % 2 == 1 ? listA : listB);
-            b.listOut = s % 2 == 0 ? listA : listB;
+            b.saved = r.saved; b.phase = r.phase;
+            b.list = s == 0 ? a.list : (s % 2 == 1 ? r.listA : r.listB);
+            b.listOut = s % 2 == 0 ? r.listA : r.listB;
             b.listLen = nActive;
-            b.listLenDev = s == 0 ? nullptr : relaunchCounts + (s - 1);
-            b.outCount = relaunchCounts + s;
+            b.listLenDev = s == 0 ? nullptr : r.counts + (s - 1);
+            b.outCount = r.counts + s;
             b.slice = s < S ? relaunchSteps : 0;
-            hipLaunchKernelGGL(fn, dim3(grid), dim3(block()), dyn_lds(), strm, b);
+            hipLaunchKernelGGL(fn, dim3(grid), dim3(B), lds, strm, b);
         }
         MPGOS_HIP_TRY(hipGetLastError());
+        return MPGOS_OK;
+    }
+
+    // The relaunch form over caller-provided device memory (`bytes` bytes at
+    // `scratch`): nothing is allocated, whatever the KernelMode option says;
+    // ThreadsPerBlock, LaunchVariant, RelaunchSteps and RelaunchLaunches apply.
+    // GraphCapture is not used (a graph instantiation allocates).
+    int solve_noalloc(void* scratch, size_t bytes) override
+    {
+        MPGOS_HIP_TRY(hipSetDevice(device));
+        if (!scratch) { set_error("mpgos_pt_solve_noalloc: no scratch memory on this device"); return MPGOS_E_STATE; }
+        if (relaunch_bytes() > bytes) {
+            set_error("mpgos_pt_solve_noalloc: the relaunch form needs " + std::to_string(relaunch_bytes()) +
+                      " bytes of scratch for NT = " + std::to_string(NT) + " (" + std::to_string(lane_words() * 4 + 12) +
+                      " per instance), more than the " + std::to_string(bytes) + " available");
+            return MPGOS_E_ARG;
+        }
+        if (nActive == 0) return MPGOS_OK;
+        const RelaunchBuffers r = carve_relaunch(scratch);
+        const Args<Sys, P, CP> a = make_args();
+        const KernelFn fn = algorithm == MPGOS_RK4 ? pick_variant<MPGOS_RK4, MPGOS_KERNEL_PLAIN + 2>(variant)
+                                                   : pick_variant<MPGOS_RKCK45, MPGOS_KERNEL_PLAIN + 2>(variant);
+        const int B = blockSize;
+        const int grid = nActive / B + (nActive % B == 0 ? 0 : 1);
+        const size_t lds = preferShared ? (size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int) : 0;
+        MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+        if (int rc = enqueue_relaunch(a, fn, grid, B, lds, r)) return rc;
+        MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
         return MPGOS_OK;
     }
 
@@ -626,7 +680,8 @@ public:
 
     int enqueue(const Args<Sys, P, CP>& a, KernelFn fn, int grid)
     {
-        if (kernelMode == MPGOS_KERNEL_RELAUNCH) return enqueue_relaunch(a, fn, grid);
+        if (kernelMode == MPGOS_KERNEL_RELAUNCH)
+            return enqueue_relaunch(a, fn, grid, block(), dyn_lds(), RelaunchBuffers{saved, phase, listA, listB, relaunchCounts});
         if (kernelMode == MPGOS_KERNEL_PERSISTENT)
             hipLaunchKernelGGL(reset_queue_kernel<0>, dim3(1), dim3(64), 0, strm, queue);
         hipLaunchKernelGGL(fn, dim3(grid), dim3(block()), dyn_lds(), strm, a);
@@ -867,6 +922,7 @@ public:
         return MPGOS_OK;
     }
     void* stream() override { return (void*)strm; }
+    int device_index() const override { return device; }
 
     // ------------------------------------------------------------ Print (PT-IF:1082-1276)
     template <class T_>

@@ -47,6 +47,7 @@ def lib():
         "mpgos_pt_sync_d2h": (c_int, [H, c_int]),
         "mpgos_pt_solve": (c_int, [H]),
         "mpgos_pt_solve_slice": (c_int, [H, ctypes.c_longlong, ctypes.POINTER(c_int)]),
+        "mpgos_pt_solve_noalloc": (c_int, [H]),
         "mpgos_pt_synchronise_device": (c_int, [H]),
         "mpgos_pt_insert_synchronisation_point": (c_int, [H]),
         "mpgos_pt_synchronise_solver": (c_int, [H]),

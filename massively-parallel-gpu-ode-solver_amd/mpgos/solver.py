@@ -175,6 +175,13 @@ class ProblemSolver:
         check(lib().mpgos_pt_solve_slice(self._h, int(max_attempted_steps), ctypes.byref(left)))
         return left.value
 
+    def solve_noalloc(self):
+        """Solve() in the relaunch form with the library's static per-GPU
+        scratch memory: allocates nothing (for graph capture and callers that
+        cannot allocate).  Bit-identical to Solve(); calls that overlap on one
+        GPU share the scratch memory and race."""
+        check(lib().mpgos_pt_solve_noalloc(self._h))
+
     def SynchroniseDevice(self):
         check(lib().mpgos_pt_synchronise_device(self._h))
 
