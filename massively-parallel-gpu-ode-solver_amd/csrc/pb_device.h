@@ -81,7 +81,9 @@ __host__ __device__ inline size_t mfma_index(int i, int s, int u, int UPS)
 
 enum Coupling { kLaneCoupling = 0, kMfmaCoupling = 1 };
 
-template <class Sys, class P>
+// CP: precision of the step size, tolerances and controller (FP64 in the
+// FLOAT_KAHAN mode, else P).
+template <class Sys, class P, class CP = P>
 struct Args {
     static constexpr int UD = Sys::UD, NE = Sys::NE;
     // unit scope, index (system * UPS + unit) + component * NU, NU = NS * UPS
@@ -94,8 +96,9 @@ struct Args {
     long long* nacc; long long* nrej; int* status;
     int NS, UPS, SPB, nActive, NDO, CBW, CCI, cMatSize, sharedGlobals, sharedMatrix;
     int doFreq;
-    P dt0, dtmax, dtmin, grow, shrink, doMinDt;
-    P rtol[UD], atol[UD];
+    CP dt0, dtmax, dtmin, grow, shrink;
+    P doMinDt;
+    CP rtol[UD], atol[UD];
     P etol[Pos<NE>::value];
     int edir[Pos<NE>::value];
     // record pass of the discrete adjoint (unused otherwise): unit states every
@@ -105,11 +108,17 @@ struct Args {
     int recK, recCkCap, recStepCap;
 };
 
-// System scope of one system (LDS).
-template <class Sys, class P>
+// System scope of one system (LDS).  KAHAN: FP32 time with its compensation
+// tc, FP64 step sizes, and dtP, the FP32 copy of the step size that the user
+// callbacks see (and may change).
+template <class Sys, class P, bool KAHAN = false>
 struct SysScope {
+    using CP = typename std::conditional<KAHAN, double, P>::type;
     P TD[2];
-    P t, dt, newDt, doT;
+    P t;
+    CP dt, newDt;
+    P doT;
+    P tc, dtP;
     P sPAR[Pos<Sys::NSP>::value];
     P sACC[Pos<Sys::NSA>::value];
     P strength[Pos<Sys::NC>::value];
@@ -143,13 +152,13 @@ struct LdsLayout {
     size_t cpt, coup, g, gi, m, total;
 };
 
-template <class Sys, class P>
+template <class Sys, class P, bool KAHAN = false>
 __host__ __device__ inline LdsLayout lds_layout(int cpl, int SPB, int UPS, int sharedGlobals, int sharedMatrix,
                                                 int cMatSize)
 {
     constexpr int NCp = Pos<Sys::NC>::value;
     LdsLayout L;
-    size_t o = (size_t)SPB * sizeof(SysScope<Sys, P>);
+    size_t o = (size_t)SPB * sizeof(SysScope<Sys, P, KAHAN>);
     o = (o + 15) & ~(size_t)15;
     L.cpt = o;
     if (cpl == kMfmaCoupling) {
@@ -174,9 +183,9 @@ __host__ __device__ inline LdsLayout lds_layout(int cpl, int SPB, int UPS, int s
 template <class Sys, class P>
 struct Unit {
     static constexpr int UD = Sys::UD, NC = Sys::NC, NE = Sys::NE;
-    using A = Args<Sys, P>;
 
     // coupling: F[CouplingIndex[i]] += strength_i CPF_i sum_col M_i(u, col) CPT_col[i]
+    template <class A>
     static __device__ __forceinline__ void couple(const A& a, const P* M, const int* cIndex, const P* strength,
                                                   const P* cptSys, int u, const P* CPF, P* F)
     {
@@ -293,18 +302,25 @@ __device__ __forceinline__ void mfma_contract44(const double* M, const double* c
 // u0 + p * (UPS / PAIRS), u0 = t % (UPS / PAIRS), so the system-scope scalars
 // (time, step) are shared by the lane's pairs and global accesses to unit-scope
 // arrays stay coalesced.
-template <class Sys, class P, int ALG, int PAIRS, int CPL, bool REC = false>
-__device__ void pb_body(const Args<Sys, P>& a)
+//
+// KAHAN (P = float): FLOAT_KAHAN mode.  Unit states, stages, right-hand sides
+// and coupling terms in FP32; the state and the time are accumulated with Kahan
+// compensation (x += dt * sum b_j k_j and t += dt as compensated sums); the
+// step size, the error norm (from the FP32 stages in FP64 arithmetic), the
+// controller and the event step control in FP64.
+template <class Sys, class P, int ALG, int PAIRS, int CPL, bool REC = false, bool KAHAN = false>
+__device__ void pb_body(const Args<Sys, P, typename SysScope<Sys, P, KAHAN>::CP>& a)
 {
     constexpr int UD = Sys::UD, NC = Sys::NC, NUP = Sys::NUP, NSP = Sys::NSP, NGP = Sys::NGP, NiGP = Sys::NiGP;
     constexpr int NUA = Sys::NUA, NiUA = Sys::NiUA, NSA = Sys::NSA, NiSA = Sys::NiSA, NE = Sys::NE;
     constexpr int NCp = Pos<NC>::value;
-    using S = SysScope<Sys, P>;
+    using S = SysScope<Sys, P, KAHAN>;
+    using CP = typename S::CP;
     using U = Unit<Sys, P>;
     using math::fma;
 
     extern __shared__ __align__(16) unsigned char mpgos_pb_lds[];
-    const LdsLayout L = lds_layout<Sys, P>(CPL, a.SPB, a.UPS, a.sharedGlobals, a.sharedMatrix, a.cMatSize);
+    const LdsLayout L = lds_layout<Sys, P, KAHAN>(CPL, a.SPB, a.UPS, a.sharedGlobals, a.sharedMatrix, a.cMatSize);
     S* sys = reinterpret_cast<S*>(mpgos_pb_lds);
     P* cptBuf = reinterpret_cast<P*>(mpgos_pb_lds + L.cpt);
     P* coupBuf = reinterpret_cast<P*>(mpgos_pb_lds + L.coup);
@@ -355,6 +371,7 @@ __device__ void pb_body(const Args<Sys, P>& a)
             s.TD[0] = a.TD[gsys]; s.TD[1] = a.TD[gsys + NS];
             s.t = a.T[gsys];
             s.dt = a.dt0; s.newDt = a.dt0;
+            s.tc = P(0); s.dtP = (P)a.dt0;
             for (int i = 0; i < NSP; i++) s.sPAR[i] = a.sPAR[gsys + i * NS];
             for (int i = 0; i < NSA; i++) s.sACC[i] = a.sACC[gsys + i * NS];
             for (int i = 0; i < NiSA; i++) s.sACCi[i] = a.sACCi[gsys + i * NS];
@@ -369,6 +386,7 @@ __device__ void pb_body(const Args<Sys, P>& a)
 
     // ---- unit scope in registers
     P X[PAIRS][UD], uPAR[PAIRS][Pos<NUP>::value], uACC[PAIRS][Pos<NUA>::value], EF[PAIRS][Pos<NE>::value];
+    P Xc[PAIRS][KAHAN ? UD : 1];                 // Kahan compensation of the unit states
     P CPF[PAIRS][NCp];
     int uACCi[PAIRS][Pos<NiUA>::value];
 #pragma unroll
@@ -376,7 +394,7 @@ __device__ void pb_body(const Args<Sys, P>& a)
         if (exists) {
             const size_t g = gu(p);
 #pragma unroll
-            for (int c = 0; c < UD; c++) X[p][c] = a.X[g + c * NU];
+            for (int c = 0; c < UD; c++) { X[p][c] = a.X[g + c * NU]; if (KAHAN) Xc[p][c] = P(0); }
 #pragma unroll
             for (int c = 0; c < NUP; c++) uPAR[p][c] = a.uPAR[g + c * NU];
 #pragma unroll
@@ -389,17 +407,22 @@ __device__ void pb_body(const Args<Sys, P>& a)
     }
     __syncthreads();
 
+    // the step size as the user callbacks see it (FP32 copy in the Kahan mode)
+    auto udt = [&]() -> P& {
+        if constexpr (KAHAN) return my->dtP;
+        else return my->dt;
+    };
 #define MPGOS_PB_USER(p, TT, DTT) TT, DTT, my->TD, X[p], uPAR[p], my->sPAR, gPAR, gPARi, uACC[p], uACCi[p], my->sACC, my->sACCi
     // ---- initialisation (PB-SOL:2465-2528)
     if (active) {
 #pragma unroll
-        for (int p = 0; p < PAIRS; p++) Sys::template Initialization<P>(gsys, uid(p), my->DOidx, MPGOS_PB_USER(p, my->t, my->dt));
+        for (int p = 0; p < PAIRS; p++) Sys::template Initialization<P>(gsys, uid(p), my->DOidx, MPGOS_PB_USER(p, my->t, udt()));
     }
     __syncthreads();
     if (NE > 0) {
         if (active) {
 #pragma unroll
-            for (int p = 0; p < PAIRS; p++) Sys::template EventFunction<P>(gsys, uid(p), EF[p], MPGOS_PB_USER(p, my->t, my->dt));
+            for (int p = 0; p < PAIRS; p++) Sys::template EventFunction<P>(gsys, uid(p), EF[p], MPGOS_PB_USER(p, my->t, udt()));
         }
         __syncthreads();
     }
@@ -478,14 +501,19 @@ __device__ void pb_body(const Args<Sys, P>& a)
         if (leader && my->term == 0) {
             my->update = 1; my->isfin = 1; my->end = 0;
             my->dt = my->newDt;
-            if (my->dt > (my->TD[1] - my->t)) { my->dt = my->TD[1] - my->t; my->end = 1; }
-            my->ratioBits = order_bits<P>(P(1e30));
+            CP remaining;
+            if (KAHAN) remaining = (CP)my->TD[1] - ((CP)my->t - (CP)my->tc);
+            else remaining = my->TD[1] - my->t;
+            if (my->dt > remaining) { my->dt = remaining; my->end = 1; }
+            if (KAHAN) my->dtP = (P)my->dt;
+            my->ratioBits = order_bits<CP>(CP(1e30));
             my->eventBits = ~0ull;            // no candidate yet (above every finite step)
             my->corrected = 0;
         }
         __syncthreads();
-        const P h = my->dt, t = my->t;
-        P next[PAIRS][UD], err[PAIRS][UD];
+        const P h = (P)my->dt, t = my->t;
+        P next[PAIRS][UD], inc[PAIRS][KAHAN ? UD : 1];
+        CP err[PAIRS][UD];
         bool isfin = true;
         if (ALG == MPGOS_RK4) {
             // PB-STP:2016-2224
@@ -511,7 +539,8 @@ __device__ void pb_body(const Args<Sys, P>& a)
             for (int p = 0; p < PAIRS; p++)
 #pragma unroll
                 for (int c = 0; c < UD; c++) {
-                    next[p][c] = fma(h * (next[p][c] + k[p][c]), P(1.0 / 6.0), X[p][c]);
+                    if (KAHAN) { inc[p][c] = h * (next[p][c] + k[p][c]) * P(1.0 / 6.0); next[p][c] = X[p][c] + inc[p][c]; }
+                    else next[p][c] = fma(h * (next[p][c] + k[p][c]), P(1.0 / 6.0), X[p][c]);
                     if (!math::isfinite(next[p][c])) isfin = false;
                 }
         } else {
@@ -552,14 +581,27 @@ __device__ void pb_body(const Args<Sys, P>& a)
             for (int p = 0; p < PAIRS; p++)
 #pragma unroll
                 for (int c = 0; c < UD; c++) {
-                    next[p][c] = fma(fma(k6[p][c], P(512.0 / 1771.0), fma(k4[p][c], P(125.0 / 594.0), fma(k3[p][c], P(250.0 / 621.0),
-                                     k1[p][c] * P(37.0 / 378.0)))), h, X[p][c]);
-                    P e = k1[p][c] * P(37.0 / 378.0 - 2825.0 / 27648.0);
-                    e = fma(k3[p][c], P(250.0 / 621.0 - 18575.0 / 48384.0), e);
-                    e = fma(k4[p][c], P(125.0 / 594.0 - 13525.0 / 55296.0), e);
-                    e = fma(k5[p][c], P(0.0 - 277.0 / 14336.0), e);
-                    e = fma(k6[p][c], P(512.0 / 1771.0 - 1.0 / 4.0), e);
-                    err[p][c] = fma(h, math::fabs(e), P(1e-18));
+                    const P sum = fma(k6[p][c], P(512.0 / 1771.0), fma(k4[p][c], P(125.0 / 594.0), fma(k3[p][c], P(250.0 / 621.0),
+                                      k1[p][c] * P(37.0 / 378.0))));
+                    if (KAHAN) {
+                        inc[p][c] = sum * h;
+                        next[p][c] = X[p][c] + inc[p][c];
+                        // FP64 error norm over the FP32 stages
+                        double e = (double)k1[p][c] * (37.0 / 378.0 - 2825.0 / 27648.0);
+                        e = math::fma((double)k3[p][c], 250.0 / 621.0 - 18575.0 / 48384.0, e);
+                        e = math::fma((double)k4[p][c], 125.0 / 594.0 - 13525.0 / 55296.0, e);
+                        e = math::fma((double)k5[p][c], 0.0 - 277.0 / 14336.0, e);
+                        e = math::fma((double)k6[p][c], 512.0 / 1771.0 - 1.0 / 4.0, e);
+                        err[p][c] = math::fma((double)h, math::fabs(e), 1e-18);
+                    } else {
+                        next[p][c] = fma(sum, h, X[p][c]);
+                        P e = k1[p][c] * P(37.0 / 378.0 - 2825.0 / 27648.0);
+                        e = fma(k3[p][c], P(250.0 / 621.0 - 18575.0 / 48384.0), e);
+                        e = fma(k4[p][c], P(125.0 / 594.0 - 13525.0 / 55296.0), e);
+                        e = fma(k5[p][c], P(0.0 - 277.0 / 14336.0), e);
+                        e = fma(k6[p][c], P(512.0 / 1771.0 - 1.0 / 4.0), e);
+                        err[p][c] = (CP)fma(h, math::fabs(e), P(1e-18));
+                    }
                     if (!math::isfinite(next[p][c]) || !math::isfinite(err[p][c])) isfin = false;
                 }
         }
@@ -567,37 +609,37 @@ __device__ void pb_body(const Args<Sys, P>& a)
         // ---- controller (PB-CTL:424-542)
         if (ALG == MPGOS_RKCK45) {
             if (lane && my->term == 0) {
-                P R = P(1e30);
+                CP R = CP(1e30);
                 int upd = 1;
 #pragma unroll
                 for (int p = 0; p < PAIRS; p++)
 #pragma unroll
                     for (int c = 0; c < UD; c++) {
-                        const P tol = dmax<P>(a.rtol[c] * dmax<P>(math::fabs(next[p][c]), math::fabs(X[p][c])), a.atol[c]);
+                        const CP tol = dmax<CP>(a.rtol[c] * dmax<CP>(math::fabs((CP)next[p][c]), math::fabs((CP)X[p][c])), a.atol[c]);
                         upd = upd & (err[p][c] < tol);
-                        R = dmin<P>(R, tol / err[p][c]);
+                        R = dmin<CP>(R, tol / err[p][c]);
                     }
                 atomicAnd(&my->update, upd);
-                atomicMin(&my->ratioBits, order_bits<P>(R));
+                atomicMin(&my->ratioBits, order_bits<CP>(R));
             }
             __syncthreads();
             if (leader && my->term == 0) {
-                const P R = from_order_bits<P>(my->ratioBits);
-                P fac = P(0.8) * math::pow_ctrl(R, my->update == 1 ? P(1.0 / 5.0) : P(1.0 / 4.0));
+                const CP R = from_order_bits<CP>(my->ratioBits);
+                CP fac = CP(0.8) * math::pow_ctrl(R, my->update == 1 ? CP(1.0 / 5.0) : CP(1.0 / 4.0));
                 if (!math::isfinite(fac)) my->isfin = 0;
                 if (my->isfin == 0) {
                     fac = a.shrink;
                     my->update = 0;
-                    if (my->dt < a.dtmin * P(1.01)) { my->term = 1; my->status |= MPGOS_STATUS_NONFINITE_MIN_STEP; }
-                } else if (my->dt < a.dtmin * P(1.01)) {
+                    if (my->dt < a.dtmin * CP(1.01)) { my->term = 1; my->status |= MPGOS_STATUS_NONFINITE_MIN_STEP; }
+                } else if (my->dt < a.dtmin * CP(1.01)) {
                     my->update = 1;
                     my->status |= MPGOS_STATUS_MIN_STEP_REACHED;
                 }
-                fac = dmin<P>(fac, a.grow);
-                fac = dmax<P>(fac, a.shrink);
-                P nd = my->dt * fac;
-                nd = dmin<P>(nd, a.dtmax);
-                nd = dmax<P>(nd, a.dtmin);
+                fac = dmin<CP>(fac, a.grow);
+                fac = dmax<CP>(fac, a.shrink);
+                CP nd = my->dt * fac;
+                nd = dmin<CP>(nd, a.dtmax);
+                nd = dmax<CP>(nd, a.dtmin);
                 my->newDt = nd;
             }
         } else {
@@ -611,14 +653,14 @@ __device__ void pb_body(const Args<Sys, P>& a)
         // ---- events (PB-SOL:2618-2655, PB-EVT:215-268)
         P EFn[PAIRS][Pos<NE>::value];
         if (NE > 0) {
-            const P tn = my->t + my->dt;
+            const P tn = my->t + (P)my->dt;
             if (lane && my->term == 0) {
                 bool corr = false;
-                P ev = my->dt;
+                CP ev = my->dt;
 #pragma unroll
                 for (int p = 0; p < PAIRS; p++) {
                     // the event function of the candidate state, at the end of the step
-                    Sys::template EventFunction<P>(gsys, uid(p), EFn[p], tn, my->newDt, my->TD, next[p], uPAR[p], my->sPAR, gPAR,
+                    Sys::template EventFunction<P>(gsys, uid(p), EFn[p], tn, (P)my->newDt, my->TD, next[p], uPAR[p], my->sPAR, gPAR,
                                                    gPARi, uACC[p], uACCi[p], my->sACC, my->sACCi);
                     if (my->update == 1) {
 #pragma unroll
@@ -626,20 +668,20 @@ __device__ void pb_body(const Args<Sys, P>& a)
                             const P tol = a.etol[i];
                             if ((EF[p][i] > tol && EFn[p][i] < -tol && a.edir[i] <= 0) ||
                                 (EF[p][i] < -tol && EFn[p][i] > tol && a.edir[i] >= 0)) {
-                                ev = dmin<P>(ev, -EF[p][i] / (EFn[p][i] - EF[p][i]) * my->dt);
+                                ev = dmin<CP>(ev, -(CP)EF[p][i] / ((CP)EFn[p][i] - (CP)EF[p][i]) * my->dt);
                                 corr = true;
                             }
                         }
                     }
                 }
                 if (corr) {
-                    atomicMin(&my->eventBits, order_bits<P>(ev));
+                    atomicMin(&my->eventBits, order_bits<CP>(ev));
                     atomicOr(&my->corrected, 1);
                 }
             }
             __syncthreads();
             if (leader && my->corrected == 1) {
-                const P ev = dmin<P>(my->dt, from_order_bits<P>(my->eventBits));   // corrected: one candidate at least
+                const CP ev = dmin<CP>(my->dt, from_order_bits<CP>(my->eventBits));   // corrected: one candidate at least
                 if (ev < a.dtmin) my->status |= MPGOS_STATUS_EVENT_OMITTED;
                 else { my->newDt = ev; my->update = 0; }
             }
@@ -651,7 +693,16 @@ __device__ void pb_body(const Args<Sys, P>& a)
 #pragma unroll
             for (int p = 0; p < PAIRS; p++)
 #pragma unroll
-                for (int c = 0; c < UD; c++) X[p][c] = next[p][c];
+                for (int c = 0; c < UD; c++) {
+                    if (KAHAN) {   // compensated x += inc
+                        const P y = inc[p][c] - Xc[p][c];
+                        const P sx = X[p][c] + y;
+                        Xc[p][c] = (sx - X[p][c]) - y;
+                        X[p][c] = sx;
+                    } else {
+                        X[p][c] = next[p][c];
+                    }
+                }
             if (REC) {
                 const long long n = my->nacc;            // accepted steps before this one
                 if (leader) {
@@ -672,12 +723,21 @@ __device__ void pb_body(const Args<Sys, P>& a)
             }
         }
         __syncthreads();
-        if (leader && upd == 1) my->t += my->dt;
+        if (leader && upd == 1) {
+            if (KAHAN) {   // compensated t += dt
+                const P y = (P)my->dt - my->tc;
+                const P st = my->t + y;
+                my->tc = (st - my->t) - y;
+                my->t = st;
+            } else {
+                my->t += my->dt;
+            }
+        }
         __syncthreads();
         if (lane && upd == 1) {
 #pragma unroll
             for (int p = 0; p < PAIRS; p++)
-                Sys::template ActionAfterSuccessfulTimeStep<P>(gsys, uid(p), my->udt, MPGOS_PB_USER(p, my->t, my->dt));
+                Sys::template ActionAfterSuccessfulTimeStep<P>(gsys, uid(p), my->udt, MPGOS_PB_USER(p, my->t, udt()));
         }
         __syncthreads();
         if (NE > 0) {
@@ -689,13 +749,13 @@ __device__ void pb_body(const Args<Sys, P>& a)
                         const P tol = a.etol[i];
                         if ((EF[p][i] > tol && math::fabs(EFn[p][i]) < tol && a.edir[i] <= 0) ||
                             (EF[p][i] < -tol && math::fabs(EFn[p][i]) < tol && a.edir[i] >= 0))
-                            Sys::template ActionAfterEventDetection<P>(gsys, uid(p), i, my->udt, MPGOS_PB_USER(p, my->t, my->dt));
+                            Sys::template ActionAfterEventDetection<P>(gsys, uid(p), i, my->udt, MPGOS_PB_USER(p, my->t, udt()));
                     }
             }
             __syncthreads();
             if (lane && upd == 1) {
 #pragma unroll
-                for (int p = 0; p < PAIRS; p++) Sys::template EventFunction<P>(gsys, uid(p), EFn[p], MPGOS_PB_USER(p, my->t, my->newDt));
+                for (int p = 0; p < PAIRS; p++) Sys::template EventFunction<P>(gsys, uid(p), EFn[p], MPGOS_PB_USER(p, my->t, (P)my->newDt));
             }
             __syncthreads();
             if (lane && upd == 1) {
@@ -755,7 +815,7 @@ __device__ void pb_body(const Args<Sys, P>& a)
     // ---- finalisation and write-back (PB-SOL:2814-2887)
     if (active) {
 #pragma unroll
-        for (int p = 0; p < PAIRS; p++) Sys::template Finalization<P>(gsys, uid(p), my->DOidx, MPGOS_PB_USER(p, my->t, my->dt));
+        for (int p = 0; p < PAIRS; p++) Sys::template Finalization<P>(gsys, uid(p), my->DOidx, MPGOS_PB_USER(p, my->t, udt()));
     }
     __syncthreads();
     if (exists) {
@@ -786,10 +846,10 @@ __device__ void pb_body(const Args<Sys, P>& a)
 
 // Lane-per-pair form; MAXT bounds the block size so that blocks of up to 256
 // threads get the larger register budget.
-template <class Sys, class P, int ALG, int MAXT>
-__global__ void __launch_bounds__(MAXT) pb_kernel(Args<Sys, P> a)
+template <class Sys, class P, int ALG, int MAXT, bool KAHAN = false>
+__global__ void __launch_bounds__(MAXT) pb_kernel(Args<Sys, P, typename SysScope<Sys, P, KAHAN>::CP> a)
 {
-    pb_body<Sys, P, ALG, 1, kLaneCoupling>(a);
+    pb_body<Sys, P, ALG, 1, kLaneCoupling, false, KAHAN>(a);
 }
 
 // Record pass of the discrete adjoint (FP64): the forward integration plus

@@ -53,9 +53,12 @@ template <class Sys, class = void> struct HasVjp : std::false_type {};
 template <class Sys>
 struct HasVjp<Sys, decltype((void)&Sys::template OdeVJP<double>)> : std::true_type {};
 
-template <class Sys, class P>
+// KAHAN (P = float): the FLOAT_KAHAN mode (compensated FP32 state and time,
+// FP64 step size, error norm, controller and event step control).
+template <class Sys, class P, bool KAHAN = false>
 class Solver : public SolverBase {
 public:
+    using CP = typename std::conditional<KAHAN, double, P>::type;
     static constexpr int UD = Sys::UD, NC = Sys::NC, NUP = Sys::NUP, NSP = Sys::NSP, NGP = Sys::NGP, NiGP = Sys::NiGP;
     static constexpr int NUA = Sys::NUA, NiUA = Sys::NiUA, NSA = Sys::NSA, NiSA = Sys::NiSA, NE = Sys::NE;
 
@@ -149,7 +152,7 @@ public:
         d->ns = NS; d->ups = UPS; d->ud = UD; d->spb = SPB; d->nc = NC; d->cbw = CBW; d->cci = CCI;
         d->nup = NUP; d->nsp = NSP; d->ngp = NGP; d->nigp = NiGP; d->nua = NUA; d->niua = NiUA; d->nsa = NSA;
         d->nisa = NiSA; d->ne = NE; d->ndo = NDO; d->algorithm = algorithm;
-        d->precision = std::is_same<P, float>::value ? MPGOS_FLOAT : MPGOS_DOUBLE;
+        d->precision = KAHAN ? MPGOS_FLOAT_KAHAN : (std::is_same<P, float>::value ? MPGOS_FLOAT : MPGOS_DOUBLE);
         return MPGOS_OK;
     }
 
@@ -399,9 +402,9 @@ public:
         }
     }
 
-    Args<Sys, P> make_args() const
+    Args<Sys, P, CP> make_args() const
     {
-        Args<Sys, P> a;
+        Args<Sys, P, CP> a;
         std::memset(&a, 0, sizeof a);
         a.X = X.d; a.uPAR = uPAR.d; a.uACC = uACC.d; a.uACCi = uACCi.d;
         a.TD = TD.d; a.T = T.d; a.sPAR = sPAR.d; a.sACC = sACC.d; a.sACCi = sACCi.d; a.strength = strength.d;
@@ -410,8 +413,14 @@ public:
         a.nacc = nacc.d; a.nrej = nrej.d; a.status = status.d;
         a.NS = NS; a.UPS = UPS; a.SPB = SPB; a.nActive = nActive; a.NDO = NDO; a.CBW = CBW; a.CCI = CCI;
         a.cMatSize = cMatSize; a.sharedGlobals = sharedGlobals; a.doFreq = doFreq;
-        a.dt0 = (P)dt0; a.dtmax = (P)dtmax; a.dtmin = (P)dtmin; a.grow = (P)grow; a.shrink = (P)shrink; a.doMinDt = (P)doMinDt;
-        for (int i = 0; i < UD; i++) { a.rtol[i] = (P)rtol[i]; a.atol[i] = (P)atol[i]; }
+        // options are stored in the solver's precision, as the reference does; FP64 in the Kahan mode
+        a.dt0 = (CP)(P)dt0; a.dtmax = (CP)(P)dtmax; a.dtmin = (CP)(P)dtmin; a.grow = (CP)(P)grow; a.shrink = (CP)(P)shrink;
+        if (KAHAN) { a.dt0 = (CP)dt0; a.dtmax = (CP)dtmax; a.dtmin = (CP)dtmin; a.grow = (CP)grow; a.shrink = (CP)shrink; }
+        a.doMinDt = (P)doMinDt;
+        for (int i = 0; i < UD; i++) {
+            a.rtol[i] = KAHAN ? (CP)rtol[i] : (CP)(P)rtol[i];
+            a.atol[i] = KAHAN ? (CP)atol[i] : (CP)(P)atol[i];
+        }
         for (int i = 0; i < NE; i++) { a.etol[i] = (P)etol[i]; a.edir[i] = edir[i]; }
         return a;
     }
@@ -432,14 +441,14 @@ public:
 
     size_t lds_bytes(int cpl, int spb, int sm) const
     {
-        return lds_layout<Sys, P>(cpl, spb, UPS, sharedGlobals, sm, cMatSize).total;
+        return lds_layout<Sys, P, KAHAN>(cpl, spb, UPS, sharedGlobals, sm, cMatSize).total;
     }
 
     // PB-IF:1949-1965
     int solve() override
     {
         MPGOS_HIP_TRY(hipSetDevice(device));
-        Args<Sys, P> a = make_args();
+        Args<Sys, P, CP> a = make_args();
         const int cpl = coupling_form();
         if (cpl == kMfmaCoupling && !mfma_applies()) {
             set_error("per-block: the dense-contraction coupling kernel needs a full FP64 coupling matrix and UPS a multiple of 32 up to 256");
@@ -455,18 +464,28 @@ public:
         const int grid = NS / spb + (NS % spb == 0 ? 0 : 1);
         lastForm = cpl;
         MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
-        if (cpl == kMfmaCoupling && mfmaPairs == 2) {
-            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RK4, 2>), dim3(grid), dim3(threads), lds, strm, a);
-            else hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RKCK45, 2>), dim3(grid), dim3(threads), lds, strm, a);
-        } else if (cpl == kMfmaCoupling) {
-            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RK4, 4>), dim3(grid), dim3(threads), lds, strm, a);
-            else hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RKCK45, 4>), dim3(grid), dim3(threads), lds, strm, a);
-        } else if (threads <= 256) {
-            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4, 256>), dim3(grid), dim3(threads), lds, strm, a);
-            else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, 256>), dim3(grid), dim3(threads), lds, strm, a);
+        if constexpr (KAHAN) {   // lane-per-pair form only
+            if (threads <= 256) {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4, 256, true>), dim3(grid), dim3(threads), lds, strm, a);
+                else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, 256, true>), dim3(grid), dim3(threads), lds, strm, a);
+            } else {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4, kMaxBlock, true>), dim3(grid), dim3(threads), lds, strm, a);
+                else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, kMaxBlock, true>), dim3(grid), dim3(threads), lds, strm, a);
+            }
         } else {
-            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a);
-            else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a);
+            if (cpl == kMfmaCoupling && mfmaPairs == 2) {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RK4, 2>), dim3(grid), dim3(threads), lds, strm, a);
+                else hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RKCK45, 2>), dim3(grid), dim3(threads), lds, strm, a);
+            } else if (cpl == kMfmaCoupling) {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RK4, 4>), dim3(grid), dim3(threads), lds, strm, a);
+                else hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RKCK45, 4>), dim3(grid), dim3(threads), lds, strm, a);
+            } else if (threads <= 256) {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4, 256>), dim3(grid), dim3(threads), lds, strm, a);
+                else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, 256>), dim3(grid), dim3(threads), lds, strm, a);
+            } else {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RK4, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a);
+                else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, kMaxBlock>), dim3(grid), dim3(threads), lds, strm, a);
+            }
         }
         MPGOS_HIP_TRY(hipGetLastError());
         MPGOS_HIP_TRY(hipEventRecord(evStop, strm));

@@ -23,6 +23,7 @@
         switch (precision) {                                                          \
         case MPGOS_DOUBLE: return new ::mpgos::pb::Solver<SYS, double>();             \
         case MPGOS_FLOAT: return new ::mpgos::pb::Solver<SYS, float>();               \
+        case MPGOS_FLOAT_KAHAN: return new ::mpgos::pb::Solver<SYS, float, true>();   \
         default: return nullptr;                                                      \
         }                                                                             \
     }
