@@ -7,8 +7,13 @@
 // control are not differentiated -- "frozen" step sequence).
 //
 // Record pass (record_body): the ordinary lane loop, plus a checkpoint
-// (x, t, proposed dt, event-function values) before accepted step 0 and after
-// every K-th accepted step, SoA in HBM: ckpt[((k*(SD+2+NE)) + c)*NT + i].
+// (x, t, proposed dt, event-function values, accessories and integer
+// accessories) before accepted step 0 and after every K-th accepted step, SoA
+// in HBM: ckpt[((k*(SD+2+NE+NA+NIA)) + c)*NT + i].  The reverse pass restores
+// the accessories with the state, so callbacks and right-hand sides that read
+// them replay the recorded values; control parameters and time domains are
+// taken as the Solve left them (callbacks that change them during a Solve are
+// outside what the adjoint differentiates).
 // Reverse pass (reverse_body), one lane per instance, segments last to first:
 //   1. restore the segment's checkpoint and re-run the same adaptive loop
 //      (controller, rejections and all) -- identical arithmetic, so the same
@@ -24,7 +29,7 @@
 // Event step control is replayed as recorded (the event values are part of
 // the checkpoint), so shortened steps recur; user actions at events are not
 // differentiated.  Cost per accepted step: one forward step (recompute) +
-// stage recompute + one VJP per stage; memory: NT * ceil(N/K) * (SD+2+NE)
+// stage recompute + one VJP per stage; memory: NT * ceil(N/K) * (SD+2+NE+NA+NIA)
 // doubles of checkpoints plus NT * K * (SD+2) of segment buffer.
 #pragma once
 
@@ -47,7 +52,7 @@ struct AdjArgs {
 };
 
 template <class Sys>
-constexpr int checkpoint_words() { return Sys::SD + 2 + Sys::NE; }
+constexpr int checkpoint_words() { return Sys::SD + 2 + Sys::NE + Sys::NA + Sys::NIA; }
 
 template <class Sys, int ALG>
 __device__ __forceinline__ void store_checkpoint(const AdjArgs<Sys>& A, const Lane<Sys, double, false, ALG>& L, int k)
@@ -61,6 +66,10 @@ __device__ __forceinline__ void store_checkpoint(const AdjArgs<Sys>& A, const La
     A.ckpt[(base + SD + 1) * NT + L.tid] = L.newDt;
 #pragma unroll
     for (int e = 0; e < NE; e++) A.ckpt[(base + SD + 2 + e) * NT + L.tid] = L.EF[e];
+#pragma unroll
+    for (int i = 0; i < Sys::NA; i++) A.ckpt[(base + SD + 2 + NE + i) * NT + L.tid] = L.ACC[i];
+#pragma unroll
+    for (int i = 0; i < Sys::NIA; i++) A.ckpt[(base + SD + 2 + NE + Sys::NA + i) * NT + L.tid] = (double)L.ACCi[i];
 }
 
 template <class Sys, int ALG>
@@ -252,10 +261,6 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
     for (int i = 0; i < 2; i++) L.TD[i] = a.TD[tid + i * NT];
 #pragma unroll
     for (int i = 0; i < NCP; i++) L.cPAR[i] = a.cPAR[tid + i * NT];
-#pragma unroll
-    for (int i = 0; i < NA; i++) L.ACC[i] = a.ACC[tid + i * NT];
-#pragma unroll
-    for (int i = 0; i < NIA; i++) L.ACCi[i] = a.ACCi[tid + i * NT];
 
     const bool ok = (a.status[tid] & MPGOS_STATUS_CHECKPOINT_OVERFLOW_BIT) == 0;
     const long long nseg = ok ? (total + A.K - 1) / A.K : 0;
@@ -267,6 +272,10 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
         L.newDt = A.ckpt[(cb + SD + 1) * NT + tid];
 #pragma unroll
         for (int e = 0; e < Sys::NE; e++) L.EF[e] = A.ckpt[(cb + SD + 2 + e) * NT + tid];
+#pragma unroll
+        for (int i = 0; i < NA; i++) L.ACC[i] = A.ckpt[(cb + SD + 2 + Sys::NE + i) * NT + tid];
+#pragma unroll
+        for (int i = 0; i < NIA; i++) L.ACCi[i] = (int)A.ckpt[(cb + SD + 2 + Sys::NE + NA + i) * NT + tid];
         L.dt = L.newDt;
         L.tc = 0.0;
         L.term = 0; L.udt = 0; L.nacc = 0; L.nrej = 0; L.status = 0; L.nev = 0;

@@ -176,3 +176,83 @@ def test_checkpoint_interval_autotuner(gpu, oracle):
         s.close()
     assert np.array_equal(lam0.view(np.uint64), base[0].view(np.uint64))
     assert np.array_equal(gp.view(np.uint64), base[1].view(np.uint64))
+
+
+def _ck45_replay(x0, k, B, ts, hs):
+    """Cash-Karp fifth-order solution of the Duffing+q system on given
+    accepted steps (t_n, h_n), vectorised over columns; zero steps leave a
+    column unchanged.  Used only to form finite differences on a frozen step
+    sequence."""
+    def f(y, t):
+        return np.stack([y[1], -k * y[1] + B * np.cos(t) - (y[0] ** 3 + y[0]), y[0] ** 2])
+    x = x0.copy()
+    for t, h in zip(ts, hs):
+        k1 = f(x, t)
+        k2 = f(x + h * (k1 / 5), t + h / 5)
+        k3 = f(x + h * (3 / 40 * k1 + 9 / 40 * k2), t + 3 / 10 * h)
+        k4 = f(x + h * (3 / 10 * k1 - 9 / 10 * k2 + 6 / 5 * k3), t + 3 / 5 * h)
+        k5 = f(x + h * (-11 / 54 * k1 + 5 / 2 * k2 - 70 / 27 * k3 + 35 / 27 * k4), t + h)
+        k6 = f(x + h * (1631 / 55296 * k1 + 175 / 512 * k2 + 575 / 13824 * k3 + 44275 / 110592 * k4 + 253 / 4096 * k5),
+               t + 7 / 8 * h)
+        x = x + h * (37 / 378 * k1 + 250 / 621 * k3 + 125 / 594 * k4 + 512 / 1771 * k6)
+    return x
+
+
+def test_finite_differences_rkck45_frozen_steps(gpu, oracle):
+    """Central differences (h = 1e-6, relative) of L = lam . x(T) on the
+    accepted-step sequence of the nominal adaptive run, replayed with the
+    step sizes held fixed: the derivative the adjoint of the adaptive Solve
+    computes (the controller is not differentiated)."""
+    nt = 32
+    spec = W.config3_duffing_adjoint(nt=nt, algorithm=Algorithms.RKCK45)
+    spec["TD"][1] = 2 * (2 * W.PI)
+    lam_T = np.array([[1.0], [0.5], [1.0]]) * np.ones((3, nt))
+    lam0, gp, out = gpu_adjoint(spec, lam_T, K=16)
+    # the accepted steps of every instance from the oracle's per-attempt trace
+    seqs = []
+    for i in range(nt):
+        tr = oracle.trace("duffing_q", spec, tid=i)
+        a = tr["accepted"] == 1
+        seqs.append((tr["t"][a], tr["h"][a]))
+        assert len(seqs[-1][0]) == out["nacc"][i]
+    N = max(len(t) for t, _ in seqs)
+    ts = np.zeros((N, nt))
+    hs = np.zeros((N, nt))
+    for i, (t, h) in enumerate(seqs):
+        ts[:len(t), i] = t
+        hs[:len(h), i] = h
+    k, B = spec["cPAR"][0], spec["cPAR"][1]
+    x0 = spec["X"].copy()
+    nominal = _ck45_replay(x0, k, B, ts, hs)
+    assert np.allclose(nominal, out["X"], rtol=1e-12, atol=1e-13)   # the replay is the recorded Solve
+    L = lambda x: (lam_T * x).sum(0)
+    for name, ad in (("k", gp[0]), ("B", gp[1]), ("x0", lam0[0]), ("v0", lam0[1])):
+        p = {"k": k, "B": B, "x0": x0[0], "v0": x0[1]}[name]
+        hstep = 1e-6 * np.abs(p).max()
+
+        def run(sign):
+            kk, BB, xx = k.copy(), B.copy(), x0.copy()
+            if name == "k": kk = kk + sign * hstep
+            elif name == "B": BB = BB + sign * hstep
+            elif name == "x0": xx[0] = xx[0] + sign * hstep
+            else: xx[1] = xx[1] + sign * hstep
+            return L(_ck45_replay(xx, kk, BB, ts, hs))
+        fd = (run(1) - run(-1)) / (2 * hstep)
+        assert rel(ad, fd) < 1e-6, (name, rel(ad, fd))
+
+
+@pytest.mark.parametrize("algorithm", [Algorithms.RK4, Algorithms.RKCK45], ids=["rk4", "rkck45"])
+def test_adjoint_of_a_system_with_accessories(gpu, oracle, algorithm):
+    """Duffing with its max |x| accessory (updated after every accepted step):
+    the checkpoints carry the accessories, the recorded Solve's outputs equal
+    the oracle's bit for bit and the gradients its sensitivities."""
+    spec = W.config0_duffing_rk4(nt=256)
+    spec["algorithm"] = int(algorithm)
+    spec["TD"][1] = 3 * (2 * W.PI)
+    if algorithm == Algorithms.RKCK45:
+        spec["rtol"] = spec["atol"] = 1e-9
+    lam_T = np.random.default_rng(11).normal(size=(2, 256))
+    lam0, gp, out = gpu_adjoint(spec, lam_T, K=5)
+    rx, rp, ref = reference_gradient(oracle, spec, lam_T)
+    assert np.array_equal(out["ACC"].view(np.uint64), ref["ACC"].view(np.uint64))
+    assert rel(lam0, rx) < 1e-9 and rel(gp, rp) < 1e-9

@@ -213,3 +213,24 @@ def test_torch_autograd_ring_period_loss(oracle, alg):
     assert np.allclose(B.grad.numpy(), fd, rtol=tol, atol=1e-9)
     # and the segment-1 part exactly against the sensitivities
     assert j1.shape[1] == 5
+
+
+def test_ring_adjoint_at_256_units(oracle):
+    """The adjoint at the per-block configuration's size (256 units per
+    system, dense coupling on the MFMA record and reverse kernels, RKCK45 at
+    1e-9) against the frozen-step forward sensitivities on a few systems."""
+    import pyoracle
+    spec = W.config4_ring(ns=8, ups=256, periods=1, system="ring_duffing_q")
+    rs = W.Stream(W.SEED, 305)
+    w = rs.normal(3 * spec["ns"] * spec["ups"]).reshape(3, -1)
+    print("ring adjoint 8 x 256 units, MFMA", flush=True)
+    lam0, gU, gS, out = _adjoint(spec, w, K=32, kernel=2)
+    ref = pyoracle.run_pb("ring_duffing_q", spec)
+    assert np.array_equal(out["X"], ref["X"]) and np.array_equal(out["nacc"], ref["nacc"])
+    jac = pyoracle.sensitivity_pb("ring_duffing_q", spec)
+    gB, gx0, gk0 = _expected(spec, jac, w)
+    ups = spec["ups"]
+    assert np.allclose(gS[0], gB[:, 0], rtol=1e-8, atol=1e-12)
+    for c in range(3):
+        assert np.allclose(lam0[c, ::ups], gx0[:, c], rtol=1e-8, atol=1e-12)
+    assert np.allclose(gU[0, ::ups], gk0[:, 0], rtol=1e-8, atol=1e-12)
