@@ -250,6 +250,9 @@ def test_device_math_matches_host_bitwise(gpu):
         L = gpu.lib()
         for name, fn in FN.items():
             arg = np.abs(x) if name in ("log", "pow", "pow_ctrl", "sqrt") else (np.clip(x, -700, 700) if name == "exp" else x)
+            if name in ("cos", "sin"):      # large arguments: up to 2^52 and the NaN range beyond
+                big = 10.0 ** rng.uniform(4, 17, 2000) * np.where(rng.uniform(size=2000) < 0.5, -1.0, 1.0)
+                arg = np.concatenate([arg[:-2000], big])
             if name in ("cos", "sin", "exp", "log", "pow", "pow_ctrl", "sqrt"):     # special arguments at the end
                 arg = np.concatenate([arg[:-len(SPECIAL)], SPECIAL])
             arg = np.ascontiguousarray(arg)
