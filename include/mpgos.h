@@ -109,7 +109,8 @@ enum {
                                            workgroup's first lane (no lane divergence on the longest serial chains);
                                            at most one per workgroup, default 0 */
     MPGOS_PT_RELAUNCH_STEPS,       /* relaunch form: attempted steps per instance per launch (default 300) */
-    MPGOS_PT_RELAUNCH_LAUNCHES,    /* relaunch form: budgeted launches before the final unbudgeted one (default 16) */
+    MPGOS_PT_RELAUNCH_LAUNCHES,    /* relaunch form: budgeted launches before the final one, which runs every instance
+                                      to its end (up to 2^31 - 1 attempted steps, MPGOS_STATUS_STEP_BUDGET beyond); default 16 */
     MPGOS_PT_NUM_OPTIONS
 };
 
@@ -134,7 +135,9 @@ enum {
     MPGOS_STATUS_NONFINITE_MIN_STEP = 2, /* RKCK45 not finite even at minimum step: terminated */
     MPGOS_STATUS_MIN_STEP_REACHED = 4,   /* step forced at the minimum size, tolerance not guaranteed */
     MPGOS_STATUS_EVENT_OMITTED = 8,      /* event step below the minimum step: detection omitted */
-    MPGOS_STATUS_CHECKPOINT_OVERFLOW = 16 /* adjoint record pass: more checkpoints than the capacity */
+    MPGOS_STATUS_CHECKPOINT_OVERFLOW = 16, /* adjoint record pass: more checkpoints than the capacity */
+    MPGOS_STATUS_STEP_BUDGET = 32        /* relaunch form: 2^31 - 1 attempted steps in its last launch; the instance
+                                            stopped there (state and time written back as they were) */
 };
 
 typedef struct mpgos_solver mpgos_solver;

@@ -59,8 +59,9 @@ struct Args {
                               // of a slice; [2] = next solo entry
     // time-sliced solve: the instances to work on are list[0, listLen) (the
     // identity 0..nActive-1 when list is null); with slice > 0 an instance
-    // that has attempted `slice` steps in this launch is suspended (slice 0:
-    // every instance runs to its end, resuming suspended ones): its lane
+    // that has attempted `slice` steps in this launch is suspended (in the
+    // relaunch form's last launch, listOut null, it stops there with
+    // MPGOS_STATUS_STEP_BUDGET instead): its lane
     // state goes to its record saved[id * W .. id * W + W) (W 32-bit words,
     // 16-byte aligned), phase[id] = 1, and its id is appended to listOut
     // (count in queue[1]); phase[id] = 2 once finished
@@ -141,7 +142,13 @@ struct Lane {
             a.phase[tid] = 2;
             return true;
         }
-        if (a.slice != 0 && --left <= 0) {   // slice 0: no budget (the relaunch form's last launch)
+        if (--left <= 0) {
+            if (!a.listOut) {   // the relaunch form's last launch: no later launch resumes the instance
+                status |= MPGOS_STATUS_STEP_BUDGET;
+                finish(a, sPAR, sPARi);
+                a.phase[tid] = 2;
+                return true;
+            }
             save(a);
             a.phase[tid] = 1;
             a.listOut[atomicAdd(a.outCount ? a.outCount : &a.queue[1], 1u)] = tid;

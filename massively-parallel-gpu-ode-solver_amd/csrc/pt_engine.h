@@ -617,11 +617,11 @@ public:
             Args<Sys, P, CP> b = a;
             b.saved = r.saved; b.phase = r.phase;
             b.list = s == 0 ? a.list : (s % 2 == 1 ? r.listA : r.listB);
-            b.listOut = s % 2 == 0 ? r.listA : r.listB;
+            b.listOut = s == S ? nullptr : (s % 2 == 0 ? r.listA : r.listB);
             b.listLen = nActive;
             b.listLenDev = s == 0 ? nullptr : r.counts + (s - 1);
             b.outCount = r.counts + s;
-            b.slice = s < S ? relaunchSteps : 0;
+            b.slice = s < S ? relaunchSteps : 0x7fffffff;
             hipLaunchKernelGGL(fn, dim3(grid), dim3(B), lds, strm, b);
         }
         MPGOS_HIP_TRY(hipGetLastError());
