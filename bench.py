@@ -26,6 +26,11 @@ Other workloads (--workload), each printed as the same JSON line:
                   coupling product on FP64 MFMA (metric: accepted
                   system-steps/s).
 
+The default run (Lorenz) also carries the rest of the metric: "adjoint" (the
+discrete adjoint of the Duffing ensemble, RK4 and RKCK45, record + reverse
+over forward with the autotuned checkpoint interval) and "ring" (the per-block
+ring on the MFMA coupling kernel with its CPU sample, parity and adjoint).
+
 Printed (one JSON line): the contract fields plus
   roofline      -- algorithmic FLOP/s of the dominant kernel (FLOPs per
                    attempted step from the oracle's op count, or the stated
@@ -54,7 +59,9 @@ import numpy as np  # noqa: E402
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
-ENGINE_NOTE = "FP64 VALU (MI355X FP64 vector and FP64 matrix peaks are both 78.6 TFLOP/s)"
+ENGINE_NOTE = "FP64 vector ALUs, no matrix instructions (MI355X FP64 vector peak 78.6 TFLOP/s)"
+ENGINE_NOTE_F32 = ("FP32 vector ALUs for the stages (MI355X FP32 vector peak 157.3 TFLOP/s); error norm, controller and "
+                   "event control in FP64")
 
 
 def log(msg):
@@ -63,6 +70,22 @@ def log(msg):
 
 def threads_here():
     return len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_record(value, unit, sample):
+    return {"value": value, "unit": unit, "cores": threads_here(), "kind": "port", "sample": sample,
+            "cpu_model": cpu_model()}
 
 
 def bitwise(a, b, keys):
@@ -195,7 +218,7 @@ def per_thread(args, mpgos, W, pyoracle):
     elif args.workload == "duffing_rk4":
         spec = W.config0_duffing_rk4(nt=args.nt or 16384)
         name = f"duffing_rk4_{spec['nt']}"
-        cpu_idx = np.arange(0, spec["nt"], 16)          # every 16th instance
+        cpu_idx = None                                  # full workload on the CPU (about 1 s on 16 cores)
         desc = {"system": "duffing", "algorithm": "RK4", "dt": "2pi/1000", "periods": 64}
     else:
         spec = W.config2_duffing_uneven(nt=args.nt or 131072)
@@ -242,9 +265,8 @@ def per_thread(args, mpgos, W, pyoracle):
         sub = spec if cpu_idx is None else W.subset(spec, cpu_idx)
         ref = pyoracle.run(spec["system"], sub, nthreads=threads_here())
         what = "full workload" if cpu_idx is None else f"{sub['nt']} of {nt} instances (every {nt // sub['nt']}th)"
-        cpu = {"value": float(ref["nacc"].sum() / ref["seconds"]), "unit": "instance-steps/s", "cores": threads_here(),
-               "kind": "port",
-               "sample": f"{what}: {int(ref['nacc'].sum())} accepted steps in {ref['seconds']:.2f} s, FP64 OpenMP oracle"}
+        cpu = cpu_record(float(ref["nacc"].sum() / ref["seconds"]), "instance-steps/s",
+                         f"{what}: {int(ref['nacc'].sum())} accepted steps in {ref['seconds']:.2f} s, FP64 OpenMP oracle")
         mine = out if cpu_idx is None else {k: (v[..., cpu_idx] if isinstance(v, np.ndarray) and v.ndim and v.shape[-1] == nt else v)
                                             for k, v in out.items()}
         if prec == 0:
@@ -260,7 +282,8 @@ def per_thread(args, mpgos, W, pyoracle):
                          {"workload": name, "instances": nt, **desc, "kernel_form": best, "launch": res["config"],
                           "accepted_steps_per_solve": accepted, "rejected_steps_per_solve": attempted - accepted,
                           "time_sliced_steps": sl or None},
-                         {"bound": "mfma", "engine": ENGINE_NOTE, "achieved": achieved,
+                         {"bound": "fp64_valu" if prec == 0 else "fp32_valu", "engine": ENGINE_NOTE if prec == 0 else ENGINE_NOTE_F32,
+                          "achieved": achieved,
                           "peak": FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                           "frac": achieved / (FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS),
                           "traffic": traffic_for(name) if prec == 0 else None, "flops_per_attempted_step": fpa, "kernel_ms": res["kernel_ms"]},
@@ -328,16 +351,14 @@ def adjoint(args, mpgos, W, pyoracle):
     achieved = fpa * attempted / (f * 1e-3) / 1e12
     cpu = None
     if not args.no_cpu:
-        idx = np.arange(0, nt, 64)
-        ref = pyoracle.run(spec["system"], W.subset(spec, idx), nthreads=threads_here())
-        cpu = {"value": float(ref["nacc"].sum() / ref["seconds"]), "unit": "instance-steps/s", "cores": threads_here(),
-               "kind": "port", "sample": f"forward pass of {len(idx)} of {nt} instances (every 64th): "
-                                         f"{int(ref['nacc'].sum())} accepted steps in {ref['seconds']:.2f} s"}
+        ref = pyoracle.run(spec["system"], spec, nthreads=threads_here())
+        cpu = cpu_record(float(ref["nacc"].sum() / ref["seconds"]), "instance-steps/s",
+                         f"forward pass of all {nt} instances: {int(ref['nacc'].sum())} accepted steps in {ref['seconds']:.2f} s")
     line = contract_line(args, accepted / (f * 1e-3), "instance-steps/s", f,
                          {"workload": f"duffing_adjoint_{args.algorithm}_{nt}", "instances": nt,
                           "algorithm": args.algorithm.upper(), "checkpoint_interval": K,
                           "accepted_steps_per_solve": accepted},
-                         {"bound": "mfma", "engine": ENGINE_NOTE, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                         {"bound": "fp64_valu", "engine": ENGINE_NOTE, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                           "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                           "flops_per_attempted_step": fpa, "kernel_ms": f},
                          cpu, None)
@@ -393,9 +414,9 @@ def ring(args, mpgos, W, pyoracle):
         idx = np.arange(0, ns, max(1, ns // 64))
         sub = W.subset_pb(spec, idx)
         ref = pyoracle.run_pb("ring_duffing", sub, nthreads=threads_here())
-        cpu = {"value": float(ref["nacc"].sum() / ref["seconds"]), "unit": "system-steps/s", "cores": threads_here(),
-               "kind": "port", "sample": f"{len(idx)} of {ns} systems (every {ns // len(idx)}th): "
-                                         f"{int(ref['nacc'].sum())} accepted system-steps in {ref['seconds']:.2f} s"}
+        cpu = cpu_record(float(ref["nacc"].sum() / ref["seconds"]), "system-steps/s",
+                         f"{len(idx)} of {ns} systems (every {ns // len(idx)}th): "
+                         f"{int(ref['nacc'].sum())} accepted system-steps in {ref['seconds']:.2f} s")
         mine = W.subset_pb_out(out, idx, U)
         parity = {"bitwise_equal_to_oracle": bool(bitwise(mine, ref, ("X", "T", "nacc", "nrej", "status"))),
                   "systems_compared": len(idx)}
@@ -484,6 +505,36 @@ def ring_adjoint(args, W, ns, U, kernel):
             "gradient_finite": bool(np.isfinite(gB).all()), **({"checkpoint_autotune": ktable} if ktable else {})}
 
 
+# --------------------------------------------------------------------------- the rest of the metric
+def headline_extras(args, mpgos, W, pyoracle):
+    """The other half of the metric, carried by the default line: the adjoint
+    over forward cost of the Duffing ensemble (RK4 and RKCK45, autotuned
+    checkpoint interval) and the per-block ring of 256 coupled units (MFMA
+    coupling kernel, its CPU sample and parity, and its adjoint), each with
+    fewer timed repetitions than its own --workload run."""
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup, sub.nt, sub.precision, sub.checkpoint_interval = 3, 1, 0, "f64", 0
+    adj = {}
+    for alg in ("rk4", "rkck45"):
+        sub.algorithm = alg
+        log(f"extras: adjoint {alg}")
+        l = adjoint(sub, mpgos, W, pyoracle)
+        a = dict(l["adjoint"])
+        a.pop("checkpoint_autotune", None)
+        adj[alg] = {"workload": l["config"]["workload"], "forward_instance_steps_per_s": l["value"],
+                    "forward_frac_fp64_peak": l["roofline"]["frac"], **a,
+                    "cpu_baseline": l["cpu_baseline"], "speedup_vs_cpu": l.get("speedup_vs_cpu")}
+    sub.form = "mfma"
+    sub.no_adjoint = False
+    log("extras: ring")
+    r = ring(sub, mpgos, W, pyoracle)
+    ring_obj = {"workload": r["config"]["workload"], "value": r["value"], "unit": r["unit"],
+                "kernel_ms": r["roofline"]["kernel_ms"], "roofline": {k: r["roofline"][k] for k in ("bound", "achieved", "peak", "unit", "frac")},
+                "cpu_baseline": r["cpu_baseline"], "speedup_vs_cpu": r.get("speedup_vs_cpu"), "parity": r["parity"],
+                "adjoint": {k: v for k, v in r.get("adjoint", {}).items() if k != "checkpoint_autotune"}}
+    return adj, ring_obj
+
+
 # --------------------------------------------------------------------------- common
 def traffic_for(name):
     tfile = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
@@ -557,6 +608,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--no-adjoint", action="store_true", help="ring: skip the per-block adjoint timing")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="lorenz: print the Lorenz line without the adjoint and ring figures of the default run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -573,6 +626,8 @@ def main():
     import pyoracle
     if args.workload in ("lorenz", "duffing_rk4", "duffing_uneven"):
         line = per_thread(args, mpgos, W, pyoracle)
+        if args.workload == "lorenz" and not args.headline_only and args.precision == "f64":
+            line["adjoint"], line["ring"] = headline_extras(args, mpgos, W, pyoracle)
     elif args.workload == "duffing_adjoint":
         line = adjoint(args, mpgos, W, pyoracle)
     else:

@@ -39,7 +39,12 @@ def test_measurement_line_fields(path):
     r = d["roofline"]
     for k in ("bound", "achieved", "peak", "unit", "frac", "traffic"):
         assert k in r, k
-    assert r["bound"] in ("hbm", "mfma") and r["unit"] in ("GB/s", "TFLOP/s")
+    # per-thread kernels issue no matrix instructions: FP64 (or FP32) vector ALU bound
+    assert r["bound"] in ("hbm", "mfma", "fp64_valu", "fp32_valu") and r["unit"] in ("GB/s", "TFLOP/s")
+    if "instances" in d["config"] and "uneven" not in d["config"]["workload"]:
+        assert r["bound"] in ("fp64_valu", "fp32_valu"), r["bound"]
+    if "systems" in d["config"]:
+        assert r["bound"] == "mfma"
     assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-9
     if r["traffic"] is not None:
         assert r["traffic"] > 0
@@ -48,3 +53,17 @@ def test_measurement_line_fields(path):
         for k in ("value", "unit", "cores", "kind", "sample"):
             assert k in c, k
         assert c["kind"] in ("port", "reference") and c["cores"] >= 1
+
+
+def test_default_line_carries_the_whole_metric():
+    """The headline line (default run) also carries the adjoint/forward cost of
+    both steppers and the per-block ring with its CPU sample and parity."""
+    with open(os.path.join(ROOT, "profiles", "bench_lorenz_rkck45_262144.json")) as f:
+        d = json.loads(f.read().strip().splitlines()[-1])
+    for alg in ("rk4", "rkck45"):
+        a = d["adjoint"][alg]
+        assert a["adjoint_over_forward"] > 0 and a["checkpoint_interval_from"] == "autotuner"
+        assert a["cpu_baseline"]["cores"] >= 1
+    r = d["ring"]
+    assert r["roofline"]["bound"] == "mfma" and r["parity"]["bitwise_equal_to_oracle"]
+    assert r["cpu_baseline"]["value"] > 0 and r["adjoint"]["adjoint_over_forward"] > 0
