@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# MPGOS_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build, liboracle_san.so, in the CPU sanitizer test)
+LIB = os.environ.get("MPGOS_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)

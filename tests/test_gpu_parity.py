@@ -297,23 +297,29 @@ def _fmt(v):
     return f"{v:18.10e}"
 
 
-def test_dropin_cpp_driver_matches_oracle(gpu, oracle, tmp_path):
+@pytest.mark.parametrize("exe_name", ["lorenz_dropin", "lorenz_dropin_san"], ids=["plain", "host_sanitized"])
+def test_dropin_cpp_driver_matches_oracle(gpu, oracle, tmp_path, exe_name):
     """A driver written against the reference's per-thread interface, with a
     dense output of 32 points at least 0.5 apart: end states bit for bit, and
     the text of its Print files (general and DenseOutput for two instances)
     character for character equal to the oracle's values written in the
-    reference's format."""
+    reference's format.  The same driver is also run with its host code (the
+    facade and the whole host engine) built with AddressSanitizer and
+    UndefinedBehaviorSanitizer, which must report nothing."""
     import os
     import subprocess
-    exe = os.path.join(os.path.dirname(__file__), "cpp", "lorenz_dropin")
-    assert os.path.exists(exe), "build() compiles tests/cpp/lorenz_dropin"
+    exe = os.path.join(os.path.dirname(__file__), "cpp", exe_name)
+    assert os.path.exists(exe), f"build() compiles tests/cpp/{exe_name}"
     nt = 4096
     spec = W.config1_lorenz(nt=nt)
     inp = np.concatenate([spec["cPAR"][0], spec["X"].ravel()]).astype(np.float64)
     (tmp_path / "in.bin").write_bytes(inp.tobytes())
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    print(f"drop-in driver {exe_name}", flush=True)
     r = subprocess.run([exe, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], capture_output=True, text=True,
-                       timeout=120, cwd=tmp_path)
-    assert r.returncode == 0, r.stderr
+                       timeout=120, cwd=tmp_path, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
     out = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.float64).reshape(5, nt)
     dspec = dict(spec)
     dspec["ndo"] = 32
