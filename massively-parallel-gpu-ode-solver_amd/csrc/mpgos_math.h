@@ -104,44 +104,51 @@ MPGOS_HD MPGOS_INLINE double sin_kernel(double r)
     return fma(r * z, s, r);
 }
 
-// r = x - n pi/2 and the quadrant n mod 4, for |x| < 2^50 (n fits an int64).
+// r = x - n pi/2 and the quadrant n mod 4, for |x| < 2^50.  n = rint(x 2/pi):
+// adding 1.5 * 2^52 rounds y = x 2/pi to an integer (to nearest, ties to even,
+// as rint) for |y| < 2^51 and leaves n mod 4 in the two lowest mantissa bits
+// (the mantissa then holds 2^51 + n).
 MPGOS_HD MPGOS_INLINE double reduce_pio2(double x, int& quadrant)
 {
-    const double n = rint(x * TWO_OVER_PI);
+    constexpr double SHIFT = 0x1.8p52;
+    const double y = x * TWO_OVER_PI;
+    const double sh = y + SHIFT;
+    const double n = sh - SHIFT;      // rint(y), exactly
     double r = fma(-n, PIO2_HI, x);   // exact: granularity of n*PIO2_HI and x is >= 2^-52 and |r| < 1
     r = fma(-n, PIO2_LO, r);
-    quadrant = (int)((int64_t)n & 3);
+    quadrant = (int)(bits(sh) & 3u);
     return r;
 }
 } // namespace detail
 
-// Arguments outside |x| < 2^50 (including +-inf and NaN) select NaN at the
-// end instead of returning early: the function is then one block of
-// arithmetic, which a compiler can evaluate once for equal arguments (RK4's
-// two mid-step stages share their time, so a right-hand side cos(T) is
-// computed once for both).
+// Arguments outside |x| < 2^50 (including +-inf and NaN) are replaced by a
+// NaN before the reduction, which then flows through to the result, instead
+// of returning early: the function is one block of arithmetic, which a
+// compiler can evaluate once for equal arguments (RK4's two mid-step stages
+// share their time, so a right-hand side cos(T) is computed once for both).
+// The result's sign is flipped by the quadrant's bit in the sign position.
 MPGOS_HD MPGOS_INLINE double cos(double x)
 {
     const bool ok = fabs(x) < 0x1p50;   // false for NaN
     int q;
-    const double r = detail::reduce_pio2(ok ? x : 0.0, q);
-    double c = detail::cos_kernel(r);
-    double s = detail::sin_kernel(r);
-    double v = (q & 1) ? s : c;
-    v = (q == 1 || q == 2) ? -v : v;
-    return ok ? v : __builtin_nan("");
+    const double r = detail::reduce_pio2(ok ? x : __builtin_nan(""), q);
+    const double c = detail::cos_kernel(r);
+    const double s = detail::sin_kernel(r);
+    const double v = (q & 1) ? s : c;
+    const uint64_t sign = (uint64_t)(((unsigned)q + 1u) & 2u) << 62;   // quadrants 1 and 2
+    return from_bits(bits(v) ^ sign);
 }
 
 MPGOS_HD MPGOS_INLINE double sin(double x)
 {
     const bool ok = fabs(x) < 0x1p50;
     int q;
-    const double r = detail::reduce_pio2(ok ? x : 0.0, q);
-    double c = detail::cos_kernel(r);
-    double s = detail::sin_kernel(r);
-    double v = (q & 1) ? c : s;
-    v = (q >= 2) ? -v : v;
-    return ok ? v : __builtin_nan("");
+    const double r = detail::reduce_pio2(ok ? x : __builtin_nan(""), q);
+    const double c = detail::cos_kernel(r);
+    const double s = detail::sin_kernel(r);
+    const double v = (q & 1) ? c : s;
+    const uint64_t sign = (uint64_t)((unsigned)q & 2u) << 62;           // quadrants 2 and 3
+    return from_bits(bits(v) ^ sign);
 }
 
 // ---------------------------------------------------------------- exp
