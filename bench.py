@@ -338,7 +338,7 @@ def adjoint(args, mpgos, W, pyoracle):
         snap.sync()
         if K == 0:                                      # checkpoint interval from the autotuner
             from mpgos.tuning import autotune_adjoint
-            K, ktable = autotune_adjoint(s, lam, steps_needed, candidates=(16, 32, 48, 64, 96, 128, 256), log=log)
+            K, ktable = autotune_adjoint(s, lam, steps_needed, candidates=(4, 6, 8, 12, 16, 24, 32, 64, 96, 128), log=log)
         cap = steps_needed // K + 2
         fwd, rec, rev = [], [], []
         for i in range(args.warmup + args.steps):

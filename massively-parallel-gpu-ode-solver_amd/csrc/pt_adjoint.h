@@ -18,7 +18,10 @@
 //   1. restore the segment's checkpoint and re-run the same adaptive loop
 //      (controller, rejections and all) -- identical arithmetic, so the same
 //      accepted steps -- storing (x_n, t_n, h_n) of each accepted step in a
-//      per-lane segment buffer seg[((j*(SD+2)) + c)*NT + i], j < K;
+//      per-lane segment buffer, in LDS when K segment records of the
+//      workgroup fit there (lds[(j*(SD+2) + c)*blockDim + lane]: the records
+//      never leave the compute unit), else in HBM seg[((j*(SD+2)) + c)*NT + i],
+//      j < K;
 //   2. walk the segment backwards: recompute the step's stage values from x_n
 //      and propagate the adjoint through the stages with the system's
 //      vector-Jacobian product (Sys::OdeVJP):
@@ -30,7 +33,8 @@
 // the checkpoint), so shortened steps recur; user actions at events are not
 // differentiated.  Cost per accepted step: one forward step (recompute) +
 // stage recompute + one VJP per stage; memory: NT * ceil(N/K) * (SD+2+NE+NA+NIA)
-// doubles of checkpoints plus NT * K * (SD+2) of segment buffer.
+// doubles of checkpoints plus K * (SD+2) doubles of segment records per lane
+// (LDS, or HBM when they do not fit).
 #pragma once
 
 #include "pt_device.h"
@@ -238,7 +242,15 @@ __device__ __forceinline__ void reverse_rkck45(int tid, int NT, const double* x,
     Vjp<Sys>::template apply<NCPA>(tid, NT, kb[0], y[0], T[0], cPAR, sPAR, sPARi, ACC, ACCi, lam, mu, gy);
 }
 
-template <class Sys, int ALG>
+// LDS bytes of the shared-parameter staging (16-byte aligned); the segment
+// records of the LDS form follow.
+template <class Sys>
+__host__ __device__ constexpr size_t adjoint_shared_bytes(int preferShared)
+{
+    return preferShared ? (((size_t)Sys::NSP * sizeof(double) + (size_t)Sys::NISP * sizeof(int)) + 15) & ~(size_t)15 : 0;
+}
+
+template <class Sys, int ALG, bool LDS_SEG>
 __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
 {
     constexpr int SD = Sys::SD, NCP = Sys::NCP, NA = Sys::NA, NIA = Sys::NIA;
@@ -248,6 +260,14 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
     const int tid = threadIdx.x + blockIdx.x * blockDim.x;
     if (tid >= a.nActive) return;
     const int NT = a.NT;
+    // segment record word w of this lane
+    extern __shared__ __align__(16) unsigned char mpgos_dyn_lds[];
+    double* const segBase = LDS_SEG ? reinterpret_cast<double*>(mpgos_dyn_lds + adjoint_shared_bytes<Sys>(a.preferShared))
+                                    : A.seg;
+    auto seg = [&](size_t w) -> double& {
+        if constexpr (LDS_SEG) return segBase[w * blockDim.x + threadIdx.x];
+        else return segBase[w * NT + tid];
+    };
     const long long total = a.nacc[tid];
     double lam[SD], mu[Pos<NCP>::value];
 #pragma unroll
@@ -292,9 +312,9 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
             if (L.nacc != before) {
                 const size_t sb = (size_t)j * (SD + 2);
 #pragma unroll
-                for (int c = 0; c < SD; c++) A.seg[(sb + c) * NT + tid] = xp[c];
-                A.seg[(sb + SD) * NT + tid] = tp;
-                A.seg[(sb + SD + 1) * NT + tid] = L.dt;
+                for (int c = 0; c < SD; c++) seg(sb + c) = xp[c];
+                seg(sb + SD) = tp;
+                seg(sb + SD + 1) = L.dt;
                 j++;
             }
         }
@@ -304,9 +324,9 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
         auto load = [&](int jj) {
             const size_t sb = (size_t)jj * (SD + 2);
 #pragma unroll
-            for (int c = 0; c < SD; c++) xq[c] = A.seg[(sb + c) * NT + tid];
-            tq = A.seg[(sb + SD) * NT + tid];
-            hq = A.seg[(sb + SD + 1) * NT + tid];
+            for (int c = 0; c < SD; c++) xq[c] = seg(sb + c);
+            tq = seg(sb + SD);
+            hq = seg(sb + SD + 1);
         };
         if (j > 0) load(j - 1);
         for (int jj = j - 1; jj >= 0; jj--) {
@@ -330,8 +350,8 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
 template <class Sys, int ALG>
 __global__ void __launch_bounds__(kMaxBlock) adjoint_record_kernel(AdjArgs<Sys> A) { record_body<Sys, ALG>(A); }
 
-template <class Sys, int ALG>
-__global__ void __launch_bounds__(kMaxBlock) adjoint_reverse_kernel(AdjArgs<Sys> A) { reverse_body<Sys, ALG>(A); }
+template <class Sys, int ALG, bool LDS_SEG>
+__global__ void __launch_bounds__(kMaxBlock) adjoint_reverse_kernel(AdjArgs<Sys> A) { reverse_body<Sys, ALG, LDS_SEG>(A); }
 
 } // namespace pt
 } // namespace mpgos

@@ -858,7 +858,8 @@ public:
         } else {
             if (K < 1 || capacity < 1) { set_error("adjoint: checkpoint interval and capacity must be >= 1"); return MPGOS_E_ARG; }
             MPGOS_HIP_TRY(hipSetDevice(device));
-            const size_t needC = (size_t)capacity * checkpoint_words<Sys>() * NT, needS = (size_t)K * (SD + 2) * NT;
+            const size_t needC = (size_t)capacity * checkpoint_words<Sys>() * NT;
+            const size_t needS = segment_in_lds(K) ? 0 : (size_t)K * (SD + 2) * NT;   // HBM segment buffer only when LDS is too small
             if (needC > ckptCount || needS > segCount) MPGOS_HIP_TRY(hipStreamSynchronize(strm));
             if (needC > ckptCount) {
                 if (ckpt) (void)hipFree(ckpt);
@@ -887,6 +888,15 @@ public:
         }
     }
 
+    // Segment records of the reverse pass in LDS: K records of SD + 2 doubles
+    // per lane of the workgroup, when they fit in 64 KiB with the staged shared
+    // parameters.
+    size_t segment_lds_bytes(int K) const { return (size_t)K * (SD + 2) * sizeof(double) * (size_t)blockSize; }
+    bool segment_in_lds(int K) const
+    {
+        return adjoint_shared_bytes<Sys>(preferShared) + segment_lds_bytes(K) <= (size_t)64 * 1024;
+    }
+
     int adjoint_reverse() override
     {
         if constexpr (!kAdjoint) {
@@ -901,10 +911,20 @@ public:
             A.a.evcap = 0;
             const int B = blockSize;
             const int grid = NT / B + (NT % B == 0 ? 0 : 1);
-            const size_t lds = preferShared ? (size_t)NSP * sizeof(P) + (size_t)NISP * sizeof(int) : 0;
+            const bool ldsSeg = segment_in_lds(adjK);
+            const size_t lds = adjoint_shared_bytes<Sys>(preferShared) + (ldsSeg ? segment_lds_bytes(adjK) : 0);
+            if (!ldsSeg && segCount < (size_t)adjK * (SD + 2) * NT) {
+                set_error("adjoint: the segment records no longer fit in LDS (ThreadsPerBlock changed since the record pass); record again");
+                return MPGOS_E_STATE;
+            }
             MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
-            if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((adjoint_reverse_kernel<Sys, MPGOS_RK4>), dim3(grid), dim3(B), lds, strm, A);
-            else hipLaunchKernelGGL((adjoint_reverse_kernel<Sys, MPGOS_RKCK45>), dim3(grid), dim3(B), lds, strm, A);
+            if (ldsSeg) {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((adjoint_reverse_kernel<Sys, MPGOS_RK4, true>), dim3(grid), dim3(B), lds, strm, A);
+                else hipLaunchKernelGGL((adjoint_reverse_kernel<Sys, MPGOS_RKCK45, true>), dim3(grid), dim3(B), lds, strm, A);
+            } else {
+                if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((adjoint_reverse_kernel<Sys, MPGOS_RK4, false>), dim3(grid), dim3(B), lds, strm, A);
+                else hipLaunchKernelGGL((adjoint_reverse_kernel<Sys, MPGOS_RKCK45, false>), dim3(grid), dim3(B), lds, strm, A);
+            }
             MPGOS_HIP_TRY(hipGetLastError());
             MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
             return MPGOS_OK;

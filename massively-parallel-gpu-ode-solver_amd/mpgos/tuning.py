@@ -193,7 +193,7 @@ def autotune_checkpoint_interval(run_record, run_reverse, candidates=(8, 16, 32,
     return min(table, key=lambda r: r["total_ms"])["K"], table
 
 
-def autotune_adjoint(solver, lam_T, max_steps, candidates=(8, 16, 32, 64, 128, 256), repeats=1, log=None):
+def autotune_adjoint(solver, lam_T, max_steps, candidates=(4, 8, 16, 32, 64, 128, 256), repeats=1, log=None):
     """autotune_checkpoint_interval for a per-thread ProblemSolver: every record
     pass starts from the inputs held when called (device snapshot); `lam_T` is
     dL/dx(T) (SD x NT) and `max_steps` the largest accepted-step count of an
