@@ -258,6 +258,9 @@ def per_thread(args, mpgos, W, pyoracle):
     if args.form in ("auto", "relaunch") and not sl:
         forms["relaunch"] = time_form(mpgos, spec, {"mode": 2, "block": 256, "variant": 2, "relaunch": 200, "launches": 32},
                                       args.steps, args.warmup, prec)
+    if args.form in ("auto", "sorted") and not sl:
+        forms["sorted"] = time_form(mpgos, spec, {"mode": 3, "block": 64, "variant": 0, "relaunch": 400},
+                                    args.steps, args.warmup, prec)
     if args.form == "auto" and not args.no_tune and not sl:
         best_cfg = tune(mpgos, spec, prec)
         forms["autotuned"] = time_form(mpgos, spec, best_cfg, args.steps, args.warmup, prec)
@@ -310,6 +313,8 @@ def per_thread(args, mpgos, W, pyoracle):
         line["persistent_over_plain"] = forms["plain"]["kernel_ms"] / forms["persistent"]["kernel_ms"]
     if "plain" in forms and "relaunch" in forms:
         line["relaunch_over_plain"] = forms["plain"]["kernel_ms"] / forms["relaunch"]["kernel_ms"]
+    if "plain" in forms and "sorted" in forms:
+        line["sorted_over_plain"] = forms["plain"]["kernel_ms"] / forms["sorted"]["kernel_ms"]
     return line
 
 
@@ -611,7 +616,7 @@ def main():
                     choices=["lorenz", "duffing_rk4", "duffing_uneven", "duffing_adjoint", "ring"])
     ap.add_argument("--nt", type=int, default=0, help="instances (systems for ring); 0 = the workload's size")
     ap.add_argument("--form", default="auto",
-                    help="auto | plain | persistent | relaunch (per-thread; duffing_uneven also: single = one "
+                    help="auto | plain | persistent | relaunch | sorted (per-thread; duffing_uneven also: single = one "
                          "plain launch, relaunch = device-chained launches of --slice-steps); auto | mfma | mfma2 | lane (ring; mfma2 = two units per lane)")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32", "f32k"])
     ap.add_argument("--algorithm", default="rk4", choices=["rk4", "rkck45"], help="duffing_adjoint only")

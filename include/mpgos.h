@@ -98,7 +98,8 @@ enum {
     MPGOS_PT_PREFER_SHARED_MEMORY,
     /* new */
     MPGOS_PT_KERNEL_MODE,          /* 0 = one lane per instance (reference form), 1 = persistent work queue with lane
-                                      compaction, 2 = one lane per instance relaunched with repacking (MPGOS_KERNEL_*) */
+                                      compaction, 2 = one lane per instance relaunched with repacking, 3 = pilot launch
+                                      then the instances ordered by estimated remaining work (MPGOS_KERNEL_*) */
     MPGOS_PT_PERSISTENT_WAVES_PER_SIMD, /* persistent grid size in waves per SIMD (1..8), default from occupancy */
     MPGOS_PT_GRAPH_CAPTURE,        /* 1 = Solve() replays a captured hipGraph of its launches */
     MPGOS_PT_EVENT_LOG_CAPACITY,   /* hits logged per instance (allocates EVENT_LOG), default 0 */
@@ -108,7 +109,8 @@ enum {
     MPGOS_PT_PERSISTENT_SOLO_INSTANCES, /* persistent form: the first N instances of the queue each run alone in a
                                            workgroup's first lane (no lane divergence on the longest serial chains);
                                            at most one per workgroup, default 0 */
-    MPGOS_PT_RELAUNCH_STEPS,       /* relaunch form: attempted steps per instance per launch (default 300) */
+    MPGOS_PT_RELAUNCH_STEPS,       /* relaunch form: attempted steps per instance per launch; sorted form: attempted
+                                      steps of the pilot launch (default 300) */
     MPGOS_PT_RELAUNCH_LAUNCHES,    /* relaunch form: budgeted launches before the final one, which runs every instance
                                       to its end (up to 2^31 - 1 attempted steps, MPGOS_STATUS_STEP_BUDGET beyond); default 16 */
     MPGOS_PT_NUM_OPTIONS
@@ -125,8 +127,13 @@ enum {
 /* MPGOS_PT_KERNEL_MODE values.  RELAUNCH: one lane per instance, relaunched
  * every RELAUNCH_STEPS attempted steps with the unfinished instances packed
  * into the lowest lanes of the whole grid (lane states suspended in HBM); all
- * launches are queued at once (asynchronous, graph-capturable). */
-enum { MPGOS_KERNEL_PLAIN = 0, MPGOS_KERNEL_PERSISTENT = 1, MPGOS_KERNEL_RELAUNCH = 2 };
+ * launches are queued at once (asynchronous, graph-capturable).  SORTED: a
+ * pilot launch of RELAUNCH_STEPS attempted steps per instance, then the
+ * unfinished instances ordered on the device by their estimated remaining
+ * steps (largest first) and run to their ends in one launch, so that each
+ * wavefront holds instances of similar length; also asynchronous and
+ * graph-capturable.  Every form gives the same bits. */
+enum { MPGOS_KERNEL_PLAIN = 0, MPGOS_KERNEL_PERSISTENT = 1, MPGOS_KERNEL_RELAUNCH = 2, MPGOS_KERNEL_SORTED = 3 };
 
 /* Per-instance status bits, replacing the reference's device printf
  * (ErrorControllers.cuh:15,66,73; EventHandling.cuh:38). */

@@ -69,6 +69,8 @@ struct Args {
     int listLen, slice;
     const unsigned* listLenDev;   // relaunch form: the list length is the previous launch's count (device)
     unsigned* outCount;           // relaunch form: count of this launch's suspended instances (else queue[1])
+    float* sortKey;               // sorted form's pilot launch: estimated remaining attempted steps of each
+                                  // suspended instance, at its slot of listOut
     int NT, nActive, NDO, evcap, preferShared;
     int doFreq;
     int roundSteps;           // persistent form: attempted steps between compactions
@@ -151,7 +153,14 @@ struct Lane {
             }
             save(a);
             a.phase[tid] = 1;
-            a.listOut[atomicAdd(a.outCount ? a.outCount : &a.queue[1], 1u)] = tid;
+            const unsigned slot = atomicAdd(a.outCount ? a.outCount : &a.queue[1], 1u);
+            a.listOut[slot] = tid;
+            if (a.sortKey) {
+                // remaining attempts at the rate so far: attempts * (TD1 - t) / (t - TD0)
+                const double done = (double)t - (double)TD[0], rest = (double)TD[1] - (double)t;
+                const double att = (double)(nacc + nrej);
+                a.sortKey[slot] = done > 0.0 ? (float)__builtin_fmin(att * rest / done, 3.0e38) : 3.0e38f;
+            }
             return true;
         }
         return false;
@@ -695,6 +704,73 @@ __global__ void reset_relaunch_kernel(int* phase, int n, unsigned* counts, int n
     const int i = threadIdx.x + blockIdx.x * blockDim.x;
     if (i < n) phase[i] = 0;
     if (i < nc) counts[i] = 0u;
+}
+
+// Sorted form: after the pilot launch the suspended instances are ordered by
+// their estimated remaining attempted steps, largest first, so that every
+// wavefront of the following launch holds instances of similar length (lane
+// coherence; the order only schedules, results are bit-identical).  A
+// counting sort on 256 buckets of 1/8 octave (float exponent and the top three
+// mantissa bits of the estimate): histogram, scan, scatter.
+__device__ __forceinline__ int sort_bucket(float key)
+{
+    const float k = key < 1.0f ? 1.0f : (key > 4.0e9f ? 4.0e9f : key);
+    const unsigned b = __float_as_uint(k);
+    const int v = ((int)(b >> 23) - 127) * 8 + (int)((b >> 20) & 7u);   // 0 .. 255 for keys in [1, 2^32)
+    return 255 - v;                                                       // largest estimates first
+}
+
+template <int Unused = 0>
+__global__ void sort_histogram_kernel(const float* key, const unsigned* n, unsigned* hist)
+{
+    __shared__ unsigned h[256];
+    for (int j = threadIdx.x; j < 256; j += blockDim.x) h[j] = 0u;
+    __syncthreads();
+    const unsigned len = *n;
+    for (unsigned i = threadIdx.x + blockIdx.x * blockDim.x; i < len; i += gridDim.x * blockDim.x)
+        atomicAdd(&h[sort_bucket(key[i])], 1u);
+    __syncthreads();
+    for (int j = threadIdx.x; j < 256; j += blockDim.x)
+        if (h[j]) atomicAdd(&hist[j], h[j]);
+}
+
+// exclusive scan of the 256 bucket counts into bucket offsets (one workgroup)
+template <int Unused = 0>
+__global__ void sort_scan_kernel(unsigned* hist)
+{
+    if (threadIdx.x == 0) {
+        unsigned run = 0u;
+        for (int j = 0; j < 256; j++) { const unsigned c = hist[j]; hist[j] = run; run += c; }
+    }
+}
+
+// one workgroup per 256 list entries: the workgroup counts its entries per
+// bucket in LDS, reserves each bucket's range with one global atomic, and
+// places its entries with LDS atomics (no global atomics per entry)
+template <int Unused = 0>
+__global__ void sort_scatter_kernel(const float* key, const int* listIn, const unsigned* n, unsigned* offset, int* listOut)
+{
+    __shared__ unsigned cnt[256], base[256];
+    const unsigned len = *n;
+    const unsigned i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (blockIdx.x * blockDim.x >= len) return;          // workgroup-uniform
+    cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const int b = i < len ? sort_bucket(key[i]) : -1;
+    if (b >= 0) atomicAdd(&cnt[b], 1u);
+    __syncthreads();
+    const unsigned c = cnt[threadIdx.x];
+    if (c) base[threadIdx.x] = atomicAdd(&offset[threadIdx.x], c);
+    cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    if (b >= 0) listOut[base[b] + atomicAdd(&cnt[b], 1u)] = listIn[i];
+}
+
+template <int Unused = 0>
+__global__ void zero_kernel(unsigned* p, int n)
+{
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i < n) p[i] = 0u;
 }
 
 // Resets the work-queue head on the stream (a kernel, so the sequence can be

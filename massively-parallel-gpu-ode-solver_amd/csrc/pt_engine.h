@@ -145,6 +145,8 @@ public:
     int queueOrder = MPGOS_QUEUE_INDEX, soloInstances = 0;
     int relaunchSteps = 300, relaunchLaunches = 16;
     unsigned* relaunchCounts = nullptr;   // per-launch suspended counts of the relaunch form
+    float* sortKey = nullptr;             // sorted form: estimated remaining steps per suspended instance
+    unsigned* sortHist = nullptr;         // sorted form: bucket counts, then offsets
     int relaunchCountsN = 0;
     int* order = nullptr;            // work-queue order of the persistent form (device), when not index order
     std::vector<int> orderHost, rankHost;
@@ -178,7 +180,8 @@ public:
         if (ckpt) (void)hipFree(ckpt);
         if (seg) (void)hipFree(seg);
         if (queue) (void)hipFree(queue);
-        for (void* q : {(void*)saved, (void*)phase, (void*)listA, (void*)listB, (void*)order, (void*)relaunchCounts})
+        for (void* q : {(void*)saved, (void*)phase, (void*)listA, (void*)listB, (void*)order, (void*)relaunchCounts,
+                        (void*)sortKey, (void*)sortHist})
             if (q) (void)hipFree(q);
         if (syncEvent) (void)hipEventDestroy(syncEvent);
         if (evStart) (void)hipEventDestroy(evStart);
@@ -262,8 +265,9 @@ public:
         case MPGOS_PT_DENSE_OUTPUT_SAVE_FREQUENCY: doFreq = (int)v; return MPGOS_OK;
         case MPGOS_PT_PREFER_SHARED_MEMORY: preferShared = (int)v != 0; return MPGOS_OK;
         case MPGOS_PT_KERNEL_MODE:
-            if ((int)v != MPGOS_KERNEL_PLAIN && (int)v != MPGOS_KERNEL_PERSISTENT && (int)v != MPGOS_KERNEL_RELAUNCH) {
-                set_error("KernelMode must be 0, 1 or 2");
+            if ((int)v != MPGOS_KERNEL_PLAIN && (int)v != MPGOS_KERNEL_PERSISTENT && (int)v != MPGOS_KERNEL_RELAUNCH &&
+                (int)v != MPGOS_KERNEL_SORTED) {
+                set_error("KernelMode must be 0, 1, 2 or 3");
                 return MPGOS_E_ARG;
             }
             kernelMode = (int)v; return MPGOS_OK;
@@ -500,7 +504,7 @@ public:
         a.nacc = nacc.d; a.nrej = nrej.d; a.status = status.d; a.nev = nev.d;
         a.evlog = evlog.d; a.evlogidx = evlogidx.d; a.queue = queue;
         a.NT = NT; a.nActive = nActive; a.NDO = NDO; a.evcap = evcap; a.preferShared = preferShared;
-        a.list = nullptr; a.listLen = nActive; a.slice = 0; a.listLenDev = nullptr; a.outCount = nullptr;
+        a.list = nullptr; a.listLen = nActive; a.slice = 0; a.listLenDev = nullptr; a.outCount = nullptr; a.sortKey = nullptr;
         a.doFreq = doFreq;
         a.roundSteps = roundSteps;
         a.solo = 0;
@@ -532,7 +536,7 @@ public:
     KernelFn kernel(bool sliced = false) const
     {
         const bool plain = kernelMode != MPGOS_KERNEL_PERSISTENT;
-        sliced = sliced || kernelMode == MPGOS_KERNEL_RELAUNCH;     // the relaunch form runs the sliced lanes
+        sliced = sliced || kernelMode == MPGOS_KERNEL_RELAUNCH || kernelMode == MPGOS_KERNEL_SORTED;   // sliced lanes
         if (algorithm == MPGOS_RK4) {
             if (sliced) return plain ? pick_variant<MPGOS_RK4, MPGOS_KERNEL_PLAIN + 2>(variant)
                                      : pick_variant<MPGOS_RK4, MPGOS_KERNEL_PERSISTENT + 2>(variant);
@@ -561,7 +565,7 @@ public:
             grid = NT / B + (NT % B == 0 ? 0 : 1);   // PT-IF:344
             return MPGOS_OK;
         }
-        if (kernelMode == MPGOS_KERNEL_RELAUNCH) {   // lists are at most nActive long
+        if (kernelMode == MPGOS_KERNEL_RELAUNCH || kernelMode == MPGOS_KERNEL_SORTED) {   // lists are at most nActive long
             grid = nActive / B + (nActive % B == 0 ? 0 : 1);
             return MPGOS_OK;
         }
@@ -628,6 +632,32 @@ public:
         return MPGOS_OK;
     }
 
+    // Sorted form: a pilot launch of relaunchSteps attempted steps per
+    // instance that also writes each suspended instance's estimated remaining
+    // steps, a counting sort of the suspended list by that estimate (largest
+    // first), and one launch that resumes the sorted instances and runs them
+    // to their ends; queued together, no host round trip.
+    int enqueue_sorted(const Args<Sys, P, CP>& a, KernelFn fn, int grid, int B, size_t lds, const RelaunchBuffers& r)
+    {
+        hipLaunchKernelGGL(reset_relaunch_kernel<0>, dim3((unsigned)(NT + 255) / 256), dim3(256), 0, strm, r.phase, NT, r.counts, 1);
+        hipLaunchKernelGGL(zero_kernel<0>, dim3(1), dim3(256), 0, strm, sortHist, 256);
+        Args<Sys, P, CP> b = a;
+        b.saved = r.saved; b.phase = r.phase;
+        b.listOut = r.listA; b.listLen = nActive; b.listLenDev = nullptr; b.outCount = r.counts;
+        b.slice = relaunchSteps; b.sortKey = sortKey;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(B), lds, strm, b);
+        const unsigned sg = (unsigned)std::min(1024, (nActive + 255) / 256);
+        hipLaunchKernelGGL(sort_histogram_kernel<0>, dim3(sg), dim3(256), 0, strm, sortKey, r.counts, sortHist);
+        hipLaunchKernelGGL(sort_scan_kernel<0>, dim3(1), dim3(64), 0, strm, sortHist);
+        hipLaunchKernelGGL(sort_scatter_kernel<0>, dim3((unsigned)(nActive + 255) / 256), dim3(256), 0, strm, sortKey, r.listA,
+                           r.counts, sortHist, r.listB);
+        b.list = r.listB; b.listOut = nullptr; b.listLenDev = r.counts; b.outCount = nullptr;
+        b.slice = 0x7fffffff; b.sortKey = nullptr;
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(B), lds, strm, b);
+        MPGOS_HIP_TRY(hipGetLastError());
+        return MPGOS_OK;
+    }
+
     // The relaunch form over caller-provided device memory (`bytes` bytes at
     // `scratch`): nothing is allocated, whatever the KernelMode option says;
     // ThreadsPerBlock, LaunchVariant, RelaunchSteps and RelaunchLaunches apply.
@@ -682,6 +712,8 @@ public:
     {
         if (kernelMode == MPGOS_KERNEL_RELAUNCH)
             return enqueue_relaunch(a, fn, grid, block(), dyn_lds(), RelaunchBuffers{saved, phase, listA, listB, relaunchCounts});
+        if (kernelMode == MPGOS_KERNEL_SORTED)
+            return enqueue_sorted(a, fn, grid, block(), dyn_lds(), RelaunchBuffers{saved, phase, listA, listB, relaunchCounts});
         if (kernelMode == MPGOS_KERNEL_PERSISTENT)
             hipLaunchKernelGGL(reset_queue_kernel<0>, dim3(1), dim3(64), 0, strm, queue);
         hipLaunchKernelGGL(fn, dim3(grid), dim3(block()), dyn_lds(), strm, a);
@@ -730,8 +762,15 @@ public:
         if (ordered()) {
             if (int rc = ensure_order()) return rc;
         }
-        if (kernelMode == MPGOS_KERNEL_RELAUNCH) {
+        if (kernelMode == MPGOS_KERNEL_RELAUNCH || kernelMode == MPGOS_KERNEL_SORTED) {
             if (int rc = ensure_slice_buffers()) return rc;
+        }
+        if (kernelMode == MPGOS_KERNEL_SORTED && !sortKey) {
+            if (hipMalloc((void**)&sortKey, sizeof(float) * (size_t)NT) != hipSuccess ||
+                hipMalloc((void**)&sortHist, sizeof(unsigned) * 256) != hipSuccess) {
+                set_error("sorted form: allocation failed");
+                return MPGOS_E_MEMORY;
+            }
         }
         Args<Sys, P, CP> a = make_args();
         if (ordered()) a.list = order;

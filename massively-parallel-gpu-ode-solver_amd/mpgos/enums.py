@@ -72,6 +72,7 @@ class KernelMode(enum.IntEnum):
     PLAIN = 0
     PERSISTENT = 1
     RELAUNCH = 2
+    SORTED = 3
 
 
 class Status(enum.IntFlag):
@@ -80,6 +81,7 @@ class Status(enum.IntFlag):
     MIN_STEP_REACHED = 4
     EVENT_OMITTED = 8
     CHECKPOINT_OVERFLOW = 16
+    STEP_BUDGET = 32
 
 
 # export the reference's bare enumerator names (RK4, TimeDomain, InitialTimeStep, ...)

@@ -5,9 +5,10 @@ Solve() calls start from identical inputs without host traffic (inputs stay
 resident in HBM).  autotune() times candidate launch configurations on the
 caller's real workload and returns the fastest:
   kernel form      plain (one lane per instance), persistent (work queue
-                   with lane compaction) or relaunch (one lane per instance,
+                   with lane compaction), relaunch (one lane per instance,
                    relaunched every RelaunchSteps steps with the unfinished
-                   instances repacked),
+                   instances repacked) or sorted (a pilot launch, then the
+                   instances ordered by estimated remaining work),
   ThreadsPerBlock  64 .. 512,
   LaunchVariant    register budget (waves-per-EU hint 1, 2, 4, 8),
   PersistentRoundSteps  attempted steps between compactions.
@@ -121,9 +122,17 @@ def default_candidates():
     plain = [{"mode": 0, "block": b, "variant": v} for b, v in itertools.product((64, 128, 256), (0, 2))]
     pers = [{"mode": 1, "block": b, "variant": v, "round": r}
             for b, v, r in itertools.product((256, 512), (0, 2), (8, 32, 128))]
-    relaunch = [{"mode": 2, "block": b, "variant": v, "relaunch": r, "launches": 32}
-                for b, v, r in itertools.product((64, 256), (0, 2), (50, 100, 200, 400))]
-    return plain + pers + relaunch
+    # (attempted steps per launch, budgeted launches): launches beyond the
+    # longest instance's need are empty dispatches, too few leave the tail to
+    # the last (unbudgeted) launch without repacking
+    shapes = [(r, n) for r in (100, 200) for n in (8, 12, 16, 32)] + [(50, 32), (400, 32)]
+    relaunch = [{"mode": 2, "block": b, "variant": v, "relaunch": r, "launches": n}
+                for b, v, (r, n) in itertools.product((64, 256), (0, 2), shapes)]
+    # sorted form: pilot launch of r attempted steps, then the instances
+    # ordered by estimated remaining work
+    srt = [{"mode": 3, "block": b, "variant": v, "relaunch": r}
+           for b, v, r in itertools.product((64, 256), (0, 2), (200, 400, 600, 800))]
+    return plain + pers + relaunch + srt
 
 
 def autotune(solver, candidates=None, repeats=2, cache=None, key=None, log=None):

@@ -77,11 +77,12 @@ import scenarios as S  # noqa: E402
 _ALL = S.all_scenarios()
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2], ids=["plain", "persistent", "relaunch"])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3], ids=["plain", "persistent", "relaunch", "sorted"])
 @pytest.mark.parametrize("name,system,spec", _ALL, ids=[x[0] for x in _ALL])
 def test_scenarios_bitwise(gpu, oracle, name, system, spec, mode):
     ref = oracle.run(system, spec)
-    out = run_gpu(spec, kernel_mode=mode, relaunch_steps=7 if mode == 2 else 0, relaunch_launches=3 if mode == 2 else None)
+    out = run_gpu(spec, kernel_mode=mode, relaunch_steps=7 if mode in (2, 3) else 0,
+                  relaunch_launches=3 if mode == 2 else None)
     keys = ("X", "T", "TD", "cPAR", "ACC", "ACCi", "nacc", "nrej", "status", "nev", "DOidx", "DOt", "DOx",
             "evlog", "evlogidx")
     assert_bitwise(out, ref, keys=keys, label=f"{name}/{mode}")
@@ -91,7 +92,7 @@ def test_scenarios_bitwise(gpu, oracle, name, system, spec, mode):
 def test_launch_shapes_do_not_change_results(gpu, oracle, block, variant):
     spec = W.config1_lorenz(nt=3000)
     ref = oracle.run("lorenz", spec)
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3):
         out = run_gpu(spec, kernel_mode=mode, block=block, variant=variant)
         assert_bitwise(out, ref, label=f"lorenz block={block} variant={variant} mode={mode}")
 
@@ -140,6 +141,25 @@ def test_relaunch_form_is_bitwise(gpu, oracle, steps, launches):
     assert_bitwise(outs[1], oracle.run("duffing_eps", cur), keys=keys, label="relaunch second solve")
 
 
+@pytest.mark.parametrize("pilot", [1, 150, 5000], ids=["pilot1", "pilot150", "pilot_beyond_all"])
+def test_sorted_form_is_bitwise(gpu, oracle, pilot):
+    """Sorted form: a pilot launch, the unfinished instances ordered on the
+    device by estimated remaining steps, one launch to the end; events, logs
+    and a ragged active count included, repeated Solves continue."""
+    full = W.config2_duffing_uneven(nt=131072, evcap=256)
+    idx = np.argsort(full["TD"][1])[:700]
+    spec = W.subset(full, idx)
+    spec["n_active"] = 650
+    print(f"sorted pilot={pilot}", flush=True)
+    ref = oracle.run("duffing_eps", spec)
+    outs = run_gpu(spec, kernel_mode=3, relaunch_steps=pilot, solves=2)
+    keys = ("X", "T", "nacc", "nrej", "status", "nev", "evlog", "evlogidx")
+    assert_bitwise(outs[0], ref, keys=keys, label="sorted")
+    cur = dict(spec)
+    cur["X"], cur["T"] = ref["X"], ref["T"]
+    assert_bitwise(outs[1], oracle.run("duffing_eps", cur), keys=keys, label="sorted second solve")
+
+
 def test_queue_order_option_is_checked(gpu):
     from mpgos import ListOfSolverOptions as O, MPGOSError, ProblemSolver
     s = ProblemSolver("lorenz", 64, 1)
@@ -154,12 +174,12 @@ def test_graph_replays_match_eager(gpu, oracle):
     from mpgos import ListOfSolverOptions as O, ListOfVariables as V, ProblemSolver
     spec = W.config1_lorenz(nt=2048)
     ref = oracle.run("lorenz", spec)
-    for mode in (0, 1, 2):
+    for mode in (0, 1, 2, 3):
         s = ProblemSolver("lorenz", spec["nt"], Algorithms.RKCK45)
         try:
             W.load(s, spec)
             s.SolverOption(O.KernelMode, mode)
-            s.SolverOption(O.RelaunchSteps, 200)         # relaunch form: several launches in the graph
+            s.SolverOption(O.RelaunchSteps, 200)         # relaunch and sorted forms: several launches in the graph
             s.SolverOption(O.RelaunchLaunches, 4)
             s.SolverOption(O.GraphCapture, 1)
             for rep in range(3):
