@@ -79,6 +79,7 @@ __device__ __forceinline__ void store_checkpoint(const AdjArgs<Sys>& A, const La
 template <class Sys, int ALG>
 __device__ __forceinline__ void record_body(const AdjArgs<Sys>& A)
 {
+    static_assert(offsetof(AdjArgs<Sys>, a) == 0, "Lane's rare paths read the Args at the start of the kernel arguments");
     const Args<Sys, double, double>& a = A.a;
     double* sPAR; int* sPARi;
     stage_shared(a, sPAR, sPARi);
@@ -254,6 +255,7 @@ template <class Sys, int ALG, bool LDS_SEG>
 __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
 {
     constexpr int SD = Sys::SD, NCP = Sys::NCP, NA = Sys::NA, NIA = Sys::NIA;
+    static_assert(offsetof(AdjArgs<Sys>, a) == 0, "Lane's rare paths read the Args at the start of the kernel arguments");
     const Args<Sys, double, double>& a = A.a;
     double* sPAR; int* sPARi;
     stage_shared(a, sPAR, sPARi);

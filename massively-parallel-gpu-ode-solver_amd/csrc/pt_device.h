@@ -82,6 +82,22 @@ struct Args {
     int edir[Pos<NE>::value];
 };
 
+// The kernel's argument block read afresh from the kernel-argument segment,
+// through a pointer the compiler cannot see through.  Used on the rare paths
+// (finishing or suspending an instance, dense-output and event-log stores):
+// the fields they read are then loaded where they are used instead of being
+// held in SGPRs, and spilled to VGPR lanes, across the whole step loop.  Valid
+// only in kernels whose argument block begins with the Args: solve_kernel and
+// the adjoint kernels (whose AdjArgs start with it, asserted there).
+template <class A>
+__device__ __forceinline__ const A& kernel_args()
+{
+    typedef __attribute__((address_space(4))) const A* KP;
+    KP p = (KP)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const A*)p;
+}
+
 template <class Sys, class P, bool KAHAN, int ALG>
 struct Lane {
     using CP = typename std::conditional<KAHAN, double, P>::type;
@@ -136,34 +152,32 @@ struct Lane {
         else load(a, id, sPAR, sPARi);
         left = a.slice;
     }
-    // after a step: true when the lane is free again (finished or suspended)
-    __device__ __forceinline__ bool settle(const A& a, P* sPAR, int* sPARi)
+    // after a step: true when the lane is to be released (finished, or its
+    // slice budget spent); release() then finishes or suspends it
+    __device__ __forceinline__ bool stop() { return term || --left <= 0; }
+    __device__ __forceinline__ void release(const A& a, P* sPAR, int* sPARi)
     {
         if (term) {
             finish(a, sPAR, sPARi);
             a.phase[tid] = 2;
-            return true;
+            return;
         }
-        if (--left <= 0) {
-            if (!a.listOut) {   // the relaunch form's last launch: no later launch resumes the instance
-                status |= MPGOS_STATUS_STEP_BUDGET;
-                finish(a, sPAR, sPARi);
-                a.phase[tid] = 2;
-                return true;
-            }
-            save(a);
-            a.phase[tid] = 1;
-            const unsigned slot = atomicAdd(a.outCount ? a.outCount : &a.queue[1], 1u);
-            a.listOut[slot] = tid;
-            if (a.sortKey) {
-                // remaining attempts at the rate so far: attempts * (TD1 - t) / (t - TD0)
-                const double done = (double)t - (double)TD[0], rest = (double)TD[1] - (double)t;
-                const double att = (double)(nacc + nrej);
-                a.sortKey[slot] = done > 0.0 ? (float)__builtin_fmin(att * rest / done, 3.0e38) : 3.0e38f;
-            }
-            return true;
+        if (!a.listOut) {   // the relaunch form's last launch: no later launch resumes the instance
+            status |= MPGOS_STATUS_STEP_BUDGET;
+            finish(a, sPAR, sPARi);
+            a.phase[tid] = 2;
+            return;
         }
-        return false;
+        save(a);
+        a.phase[tid] = 1;
+        const unsigned slot = atomicAdd(a.outCount ? a.outCount : &a.queue[1], 1u);
+        a.listOut[slot] = tid;
+        if (a.sortKey) {
+            // remaining attempts at the rate so far: attempts * (TD1 - t) / (t - TD0)
+            const double done = (double)t - (double)TD[0], rest = (double)TD[1] - (double)t;
+            const double att = (double)(nacc + nrej);
+            a.sortKey[slot] = done > 0.0 ? (float)__builtin_fmin(att * rest / done, 3.0e38) : 3.0e38f;
+        }
     }
 
     // ------------------------------------------------------------ load / prologue
@@ -435,12 +449,13 @@ struct Lane {
                     const P tol = a.etol[i];
                     if ((EF[i] > tol && math::fabs(EFn[i]) < tol && a.edir[i] <= 0) ||
                         (EF[i] < -tol && math::fabs(EFn[i]) < tol && a.edir[i] >= 0)) {
-                        if (nev < a.evcap) {
+                        const A& c = kernel_args<A>();   // rare path: fields loaded here
+                        if (nev < c.evcap) {
                             const long long base = (long long)nev * (SD + 1);
-                            a.evlog[tid + base * NT] = t;
+                            c.evlog[tid + base * NT] = t;
 #pragma unroll
-                            for (int c = 0; c < SD; c++) a.evlog[tid + (base + 1 + c) * NT] = X[c];
-                            a.evlogidx[tid + (long long)nev * NT] = i;
+                            for (int j = 0; j < SD; j++) c.evlog[tid + (base + 1 + j) * NT] = X[j];
+                            c.evlogidx[tid + (long long)nev * NT] = i;
                         }
                         nev++;
                         Sys::template ActionAfterEventDetection<P>(tid, NT, i, udt, t, dtP, TD, X, cPAR, sPAR, sPARi, ACC, ACCi);
@@ -456,7 +471,7 @@ struct Lane {
                 if (doIdx < a.NDO && doT < t && skipped >= a.doFreq - 1) upd = 1;
                 else upd = 0;
                 if (doIdx < a.NDO && (endReached == 1 || udt == 1)) upd = 1;
-                if (upd) store_dense(a);
+                if (upd) store_dense(kernel_args<A>());
                 else skipped++;
             }
             if (endReached == 1 || udt == 1) term = 1;
@@ -539,18 +554,31 @@ __device__ __forceinline__ void plain_body(const Args<Sys, P, typename Lane<Sys,
     const int i = threadIdx.x + blockIdx.x * blockDim.x;
     const int n = (SLICED && a.listLenDev) ? (int)*a.listLenDev : a.listLen;
     if (i < n) {
-        Lane<Sys, P, KAHAN, ALG> L;
+        using L_t = Lane<Sys, P, KAHAN, ALG>;
+        L_t L;
         const int id = a.list ? a.list[i] : i;
+        // With event functions the step keeps more uniform values live (event
+        // tolerances, direction tests) than there are SGPRs, and the
+        // compiler spills argument fields to VGPR lanes and reloads them as
+        // whole 16-register tuples inside the loop; re-reading the argument
+        // block at every step instead keeps the loop free of spills (fewer
+        // instructions per step, which bounds a lone lane).  Without events
+        // the loop does not spill and keeps its hoisted arguments.
+        auto step_args = [&]() -> const typename L_t::A& {
+            if constexpr (Sys::NE > 0) return kernel_args<typename L_t::A>();
+            else return a;
+        };
         if constexpr (SLICED) {
+            // the loop only steps; finishing or suspending follows it, so
+            // the step loop keeps the plain form's registers
             L.begin(a, id, sPAR, sPARi);
-            for (;;) {
-                L.step(a, sPAR, sPARi);
-                if (L.settle(a, sPAR, sPARi)) break;
-            }
+            do L.step(step_args(), sPAR, sPARi);
+            while (!L.stop());
+            L.release(kernel_args<typename L_t::A>(), sPAR, sPARi);
         } else {
             L.load(a, id, sPAR, sPARi);
-            while (L.term == 0) L.step(a, sPAR, sPARi);
-            L.finish(a, sPAR, sPARi);
+            while (L.term == 0) L.step(step_args(), sPAR, sPARi);
+            L.finish(kernel_args<typename L_t::A>(), sPAR, sPARi);
         }
     }
 }
@@ -679,9 +707,12 @@ __device__ __forceinline__ void persistent_body(const Args<Sys, P, typename Lane
             if (busy) {
                 L.step(a, sPAR, sPARi);
                 if constexpr (SLICED) {
-                    if (L.settle(a, sPAR, sPARi)) busy = false;
+                    if (L.stop()) {
+                        L.release(kernel_args<typename L_t::A>(), sPAR, sPARi);
+                        busy = false;
+                    }
                 } else if (L.term) {
-                    L.finish(a, sPAR, sPARi);
+                    L.finish(kernel_args<typename L_t::A>(), sPAR, sPARi);
                     busy = false;
                 }
             }
