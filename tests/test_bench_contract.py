@@ -24,7 +24,7 @@ def test_more_than_one_gpu_is_refused():
 def test_there_are_measurement_lines():
     names = {os.path.basename(p) for p in LINES}
     assert {"bench_lorenz_rkck45_262144.json", "bench_ring_duffing_4096x256.json",
-            "bench_duffing_uneven_relaunch.json", "bench_adjoint_rk4.json"} <= names
+            "bench_duffing_uneven_relaunch.json", "bench_duffing_uneven_sorted.json", "bench_adjoint_rk4.json"} <= names
 
 
 @pytest.mark.parametrize("path", LINES, ids=[os.path.basename(p) for p in LINES])
@@ -41,7 +41,7 @@ def test_measurement_line_fields(path):
         assert k in r, k
     # per-thread kernels issue no matrix instructions: FP64 (or FP32) vector ALU bound
     assert r["bound"] in ("hbm", "mfma", "fp64_valu", "fp32_valu") and r["unit"] in ("GB/s", "TFLOP/s")
-    if "instances" in d["config"] and "uneven" not in d["config"]["workload"]:
+    if "instances" in d["config"]:
         assert r["bound"] in ("fp64_valu", "fp32_valu"), r["bound"]
     if "systems" in d["config"]:
         assert r["bound"] == "mfma"
