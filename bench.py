@@ -14,9 +14,12 @@ Other workloads (--workload), each printed as the same JSON line:
   duffing_uneven  131072 Duffing oscillators with forcing frequency
                   eps ~ 10^U[-3,0] and horizons of 16..4096 periods, RKCK45 at
                   1e-9, event x' = 0 with up to 256 logged hits; the longest
-                  instances take ~6e7 steps, so the Solve runs time-sliced
-                  (--slice-steps per launch, bit-identical to one Solve) and
-                  reports progress after each slice;
+                  instances take ~6e7 steps, so the default run is
+                  host-sliced (--slice-steps per launch, bit-identical to one
+                  Solve, progress after each slice); --form single | relaunch
+                  | sorted time one Solve of the reference's one-launch form,
+                  the relaunch form or the sorted form, with progress read
+                  from the device while it runs;
   duffing_adjoint 65536 Duffing oscillators with a quadrature state, RK4 at
                   dt = 2pi/500 (or --algorithm rkck45 at 1e-9): forward Solve
                   and discrete adjoint (record + reverse), reporting the
@@ -28,7 +31,9 @@ Other workloads (--workload), each printed as the same JSON line:
 
 The default run (Lorenz) also carries the rest of the metric: "adjoint" (the
 discrete adjoint of the Duffing ensemble, RK4 and RKCK45, record + reverse
-over forward with the autotuned checkpoint interval) and "ring" (the per-block
+over forward with the autotuned checkpoint interval; the record pass is itself
+the forward Solve with checkpoints, so "gradient_run_extra_over_forward" is
+what a gradient adds to the forward Solve it replaces) and "ring" (the per-block
 ring on the MFMA coupling kernel with its CPU sample, parity and adjoint).
 
 Printed (one JSON line): the contract fields plus
@@ -387,7 +392,8 @@ def adjoint(args, mpgos, W, pyoracle):
                           "flops_per_attempted_step": fpa, "kernel_ms": f},
                          cpu, None)
     line["adjoint"] = {"forward_ms": f, "record_ms": r, "reverse_ms": b, "adjoint_over_forward": (r + b) / f,
-                       "reverse_over_forward": b / f, "checkpoint_interval": K,
+                       "reverse_over_forward": b / f, "gradient_run_extra_over_forward": (r + b - f) / f,
+                       "checkpoint_interval": K,
                        "checkpoint_interval_from": "autotuner" if ktable else "command line"}
     if ktable:
         line["adjoint"]["checkpoint_autotune"] = ktable
@@ -526,6 +532,7 @@ def ring_adjoint(args, W, ns, U, kernel):
             "checkpoint_interval": K, "checkpoint_interval_from": "autotuner" if ktable else "command line",
             "forward_ms": f, "record_ms": r, "reverse_ms": b,
             "adjoint_over_forward": (r + b) / f, "reverse_over_forward": b / f,
+            "gradient_run_extra_over_forward": (r + b - f) / f,
             "gradient_finite": bool(np.isfinite(gB).all()), **({"checkpoint_autotune": ktable} if ktable else {})}
 
 

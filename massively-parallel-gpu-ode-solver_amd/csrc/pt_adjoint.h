@@ -22,7 +22,9 @@
 //      workgroup fit there (lds[(j*(SD+2) + c)*blockDim + lane]: the records
 //      never leave the compute unit), else in HBM seg[((j*(SD+2)) + c)*NT + i],
 //      j < K;
-//   2. walk the segment backwards: recompute the step's stage values from x_n
+//   2. walk the segment backwards (while the next segment down is re-stepped
+//      into the record slots already read: two independent chains in one
+//      loop body, see reverse_body): recompute the step's stage values from x_n
 //      and propagate the adjoint through the stages with the system's
 //      vector-Jacobian product (Sys::OdeVJP):
 //        x_{n+1} = x_n + h sum_j b_j k_j,  k_j = f(y_j, t_n + c_j h),
@@ -286,7 +288,9 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
 
     const bool ok = (a.status[tid] & MPGOS_STATUS_CHECKPOINT_OVERFLOW_BIT) == 0;
     const long long nseg = ok ? (total + A.K - 1) / A.K : 0;
-    for (long long s = nseg - 1; s >= 0; s--) {
+    const int K = A.K;
+    // Restore segment s's checkpoint into the lane.
+    auto restore = [&](long long s) {
         const size_t cb = (size_t)s * checkpoint_words<Sys>();
 #pragma unroll
         for (int c = 0; c < SD; c++) L.X[c] = A.ckpt[(cb + c) * NT + tid];
@@ -302,45 +306,89 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
         L.tc = 0.0;
         L.term = 0; L.udt = 0; L.nacc = 0; L.nrej = 0; L.status = 0; L.nev = 0;
         L.doIdx = 0; L.skipped = 0; L.doT = 0.0;
-        const int cnt = (int)((total - s * A.K) < A.K ? (total - s * A.K) : A.K);
-        int j = 0;
-        while (j < cnt && L.term == 0) {
-            double xp[SD];
+    };
+    // accepted steps of segment s
+    auto count = [&](long long s) { return (int)((total - s * K) < K ? (total - s * K) : K); };
+    // One record buffer of K slots serves two segments at once: segment s is
+    // differentiated (its records read last to first) while segment s - 1 is
+    // re-stepped into the slots already read.  Segments alternate the slot
+    // order -- step j in slot j, or in slot K - 1 - j -- so that the m-th
+    // record written (step m of segment s - 1) goes to the slot the m-th read
+    // (step cnt - 1 - m of segment s) has just freed; rejected attempts write
+    // nothing, so writes never overtake reads.
+    auto slot = [&](bool flip, int j) { return flip ? K - 1 - j : j; };
+    // One attempted re-step of the lane; an accepted step stores (x_n, t_n, h_n)
+    // of step j in its slot and advances j.
+    auto attempt = [&](bool flip, int& j) {
+        double xp[SD];
 #pragma unroll
-            for (int c = 0; c < SD; c++) xp[c] = L.X[c];
-            const double tp = L.t;
-            const int before = L.nacc;
-            L.step(a, sPAR, sPARi);
-            if (L.nacc != before) {
-                const size_t sb = (size_t)j * (SD + 2);
+        for (int c = 0; c < SD; c++) xp[c] = L.X[c];
+        const double tp = L.t;
+        const int before = L.nacc;
+        // the Cash-Karp step with its controller holds more uniform values
+        // than fit in SGPRs next to the adjoint chain: re-read the argument
+        // block at every attempt instead of spilling it
+        if constexpr (ALG == MPGOS_RKCK45) L.template step<false>(kernel_args<Args<Sys, double, double>>(), sPAR, sPARi);
+        else L.template step<false>(a, sPAR, sPARi);
+        if (L.nacc != before) {
+            const size_t sb = (size_t)slot(flip, j) * (SD + 2);
 #pragma unroll
-                for (int c = 0; c < SD; c++) seg(sb + c) = xp[c];
-                seg(sb + SD) = tp;
-                seg(sb + SD + 1) = L.dt;
-                j++;
-            }
+            for (int c = 0; c < SD; c++) seg(sb + c) = xp[c];
+            seg(sb + SD) = tp;
+            seg(sb + SD + 1) = L.dt;
+            j++;
         }
-        // walk back through the segment; step jj - 1's record is loaded while
-        // step jj is differentiated (hides the memory latency)
-        double xq[SD], tq = 0.0, hq = 0.0;
-        auto load = [&](int jj) {
-            const size_t sb = (size_t)jj * (SD + 2);
+    };
+    // Adjoint through step j of the segment whose records use slot order flip,
+    // with the parameters and accessories the lane held at the end of that
+    // segment's re-step (kept apart while the next segment is re-stepped)
+    double bPAR[Pos<NCP>::value], bACC[Pos<NA>::value];
+    int bACCi[Pos<NIA>::value];
+    auto keep = [&]() {
 #pragma unroll
-            for (int c = 0; c < SD; c++) xq[c] = seg(sb + c);
-            tq = seg(sb + SD);
-            hq = seg(sb + SD + 1);
-        };
-        if (j > 0) load(j - 1);
-        for (int jj = j - 1; jj >= 0; jj--) {
-            double x[SD];
+        for (int i = 0; i < NCP; i++) bPAR[i] = L.cPAR[i];
 #pragma unroll
-            for (int c = 0; c < SD; c++) x[c] = xq[c];
-            const double tn = tq, hn = hq;
-            if (jj > 0) load(jj - 1);
-            if (ALG == MPGOS_RK4)
-                reverse_rk4<Sys>(tid, NT, x, tn, hn, L.cPAR, sPAR, sPARi, L.ACC, L.ACCi, lam, mu);
-            else
-                reverse_rkck45<Sys>(tid, NT, x, tn, hn, L.cPAR, sPAR, sPARi, L.ACC, L.ACCi, lam, mu);
+        for (int i = 0; i < NA; i++) bACC[i] = L.ACC[i];
+#pragma unroll
+        for (int i = 0; i < NIA; i++) bACCi[i] = L.ACCi[i];
+    };
+    auto back = [&](bool flip, int j) {
+        const size_t sb = (size_t)slot(flip, j) * (SD + 2);
+        double x[SD];
+#pragma unroll
+        for (int c = 0; c < SD; c++) x[c] = seg(sb + c);
+        const double tn = seg(sb + SD), hn = seg(sb + SD + 1);
+        if (ALG == MPGOS_RK4)
+            reverse_rk4<Sys>(tid, NT, x, tn, hn, bPAR, sPAR, sPARi, bACC, bACCi, lam, mu);
+        else
+            reverse_rkck45<Sys>(tid, NT, x, tn, hn, bPAR, sPAR, sPARi, bACC, bACCi, lam, mu);
+    };
+    if (nseg > 0) {
+        // the last segment alone
+        bool flip = false;
+        restore(nseg - 1);
+        int have = 0;
+        const int cntLast = count(nseg - 1);
+        while (have < cntLast && L.term == 0) attempt(flip, have);
+        for (long long s = nseg - 1; s >= 0; s--) {
+            // `have` records of segment s are in the buffer (slot order flip)
+            keep();
+            if (s == 0) {
+                for (int jj = have - 1; jj >= 0; jj--) back(flip, jj);
+                break;
+            }
+            // segment s - 1 is full (K accepted steps, at least K attempts),
+            // so the first `have` <= K iterations always hold one attempt and
+            // one adjoint step: two independent chains in one block
+            restore(s - 1);
+            int next = 0;
+            for (int m = 0; m < have; m++) {
+                back(flip, have - 1 - m);
+                attempt(!flip, next);
+            }
+            while (next < K && L.term == 0) attempt(!flip, next);
+            have = next;
+            flip = !flip;
         }
     }
 #pragma unroll

@@ -400,7 +400,10 @@ struct Lane {
     }
 
     // ------------------------------------------------------------ one attempted step
-    // One iteration of the loop at PT-SOL:197-394.
+    // One iteration of the loop at PT-SOL:197-394.  DENSE = false compiles the
+    // dense-output bookkeeping out (the adjoint's re-steps, which run with
+    // NDO = 0), so that the step has no branch on it.
+    template <bool DENSE = true>
     __device__ __forceinline__ void step(const A& a, P* sPAR, int* sPARi)
     {
         const int NT = a.NT;
@@ -465,7 +468,7 @@ struct Lane {
 #pragma unroll
                 for (int i = 0; i < NE; i++) EF[i] = EFn[i];
             }
-            if (a.NDO > 0) {
+            if (DENSE && a.NDO > 0) {
                 // PT-DO:40-58 then PT-DO:5-37
                 int upd;
                 if (doIdx < a.NDO && doT < t && skipped >= a.doFreq - 1) upd = 1;
