@@ -406,7 +406,8 @@ const char* mpgos_pb_list_systems(void);
 
 /* ---------------------------------------------------------- math probes
  * Evaluate the library's elementary functions (0 cos, 1 sin, 2 exp, 3 log,
- * 4 pow(x, y), 5 sqrt, 6 the controllers' power pow_ctrl(x, y)) on the host, or on the device over device buffers, so tests can
+ * 4 pow(x, y), 5 sqrt, 6 the per-block controller's power pow_ctrl(x, y), 7 the per-thread controller's root
+ * ctrl_root(x, y != 0): x^(1/5) when y != 0, else x^(1/4)) on the host, or on the device over device buffers, so tests can
  * show that both sides produce the same bits. */
 int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, long n);
 int mpgos_math_eval_device(int fn, const double* x_dev, const double* y_dev, double* out_dev, long n);

@@ -50,6 +50,7 @@ __global__ void math_eval_kernel(int fn, const double* x, const double* y, doubl
     case 3: out[i] = math::log(x[i]); break;
     case 5: out[i] = math::sqrt(x[i]); break;
     case 6: out[i] = math::pow_ctrl(x[i], y[i]); break;
+    case 7: out[i] = math::ctrl_root(x[i], y[i] != 0.0); break;
     default: out[i] = math::pow(x[i], y[i]); break;
     }
 }
@@ -330,7 +331,7 @@ void* mpgos_pb_stream(mpgos_solver* h)
 
 int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, long n)
 {
-    if (fn < 0 || fn > 6 || !x || !out || ((fn == 4 || fn == 6) && !y)) { set_error("mpgos_math_eval_host: bad arguments"); return MPGOS_E_ARG; }
+    if (fn < 0 || fn > 7 || !x || !out || ((fn == 4 || fn >= 6) && !y)) { set_error("mpgos_math_eval_host: bad arguments"); return MPGOS_E_ARG; }
     for (long i = 0; i < n; i++) {
         switch (fn) {
         case 0: out[i] = mpgos::math::cos(x[i]); break;
@@ -339,6 +340,7 @@ int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, 
         case 3: out[i] = mpgos::math::log(x[i]); break;
         case 5: out[i] = mpgos::math::sqrt(x[i]); break;
         case 6: out[i] = mpgos::math::pow_ctrl(x[i], y[i]); break;
+        case 7: out[i] = mpgos::math::ctrl_root(x[i], y[i] != 0.0); break;
         default: out[i] = mpgos::math::pow(x[i], y[i]); break;
         }
     }
@@ -347,7 +349,7 @@ int mpgos_math_eval_host(int fn, const double* x, const double* y, double* out, 
 
 int mpgos_math_eval_device(int fn, const double* x_dev, const double* y_dev, double* out_dev, long n)
 {
-    if (fn < 0 || fn > 6 || !x_dev || !out_dev || ((fn == 4 || fn == 6) && !y_dev) || n < 0) {
+    if (fn < 0 || fn > 7 || !x_dev || !out_dev || ((fn == 4 || fn >= 6) && !y_dev) || n < 0) {
         set_error("mpgos_math_eval_device: bad arguments");
         return MPGOS_E_ARG;
     }

@@ -315,6 +315,62 @@ MPGOS_HD MPGOS_INLINE double pow_ctrl(double x, double y)
     return x != x ? x + y : v;
 }
 
+// ---------------------------------------------------------------- controller roots
+// The per-thread controllers' factor R^(1/5) (accepted step) and R^(1/4)
+// (rejected step), R >= 0, from roots instead of exp(y log R):
+//  * R^(1/4) = sqrt(sqrt(R)), two correctly rounded square roots (< 1 ulp);
+//  * R^(1/5): R = 2^e m, e = 5q + r (0 <= r < 5), z = 2^r m in [1, 32);
+//    w ~ z^(-1/5) starts from a quartic in m (m^(-1/5) on [1, 2], relative
+//    error 2.1e-5) times a quartic in r through the five values 2^(-r/5),
+//    then two Newton steps w += w (1 - z w^5) / 5 (error 3 eps^2 per step);
+//    z^(1/5) = z w^4 corrected by its residual, scaled by 2^q: a few ulp.
+// Only IEEE operations and integer arithmetic, so host and device agree bit
+// for bit.  Subnormal R is scaled by 2^1000 first; 0 gives 0, +inf gives
+// +inf, NaN gives NaN.
+namespace detail {
+MPGOS_HD MPGOS_INLINE double root5(double x)
+{
+    const bool tiny = x < 0x1p-1000;
+    const double xs = tiny ? x * 0x1p1000 : x;
+    const uint64_t b = bits(xs);
+    const int e = (int)(b >> 52) - 1023;
+    const double m = from_bits((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    const int q = (e + 1025) / 5 - 205;          // floor(e / 5)
+    const int r = e - 5 * q;                     // 0 .. 4
+    const double z = m * pow2i(r);
+    double pm = 0x1.d2cadb0bb3296p-7;            // m^(-1/5), Chebyshev fit on [1, 2]
+    pm = fma(pm, m, -0x1.ca9fab0a0d793p-4);
+    pm = fma(pm, m, 0x1.714a36f55ea3cp-2);
+    pm = fma(pm, m, -0x1.484ddb53b357dp-1);
+    pm = fma(pm, m, 0x1.60d7515a3c0e5p+0);
+    const double rd = (double)r;
+    double pr = 0x1.88970ec6f3ab9p-17;           // 2^(-r/5) at r = 0 .. 4
+    pr = fma(pr, rd, -0x1.c4b4e3b1a9125p-12);
+    pr = fma(pr, rd, 0x1.3a4e88f4a3e7ap-7);
+    pr = fma(pr, rd, -0x1.1be579c2b43dbp-3);
+    pr = fma(pr, rd, 1.0);
+    double w = pm * pr;
+    for (int i = 0; i < 2; i++) {
+        const double w2 = w * w;
+        const double w5 = (w2 * w2) * w;
+        w = fma(0.2 * w, fma(-z, w5, 1.0), w);
+    }
+    const double w2 = w * w, w4 = w2 * w2;
+    const double y = z * w4;
+    const double d = fma(-z, w4 * w, 1.0);      // w = z^(-1/5) (1 - d / 5): y = z^(1/5) (1 - 4 d / 5)
+    double v = fma(0.8 * d, y, y) * pow2i(tiny ? q - 200 : q);
+    v = x == 0.0 ? 0.0 : v;
+    v = x > 0x1.fffffffffffffp+1023 ? x : v;
+    return x != x ? x : v;
+}
+} // namespace detail
+
+MPGOS_HD MPGOS_INLINE double ctrl_root(double x, bool fifth)
+{
+    if (fifth) return detail::root5(x);
+    return sqrt(sqrt(x));
+}
+
 // ---------------------------------------------------------------- float versions
 // Used by the FP32 stepping mode: computed through the double routines and
 // rounded once, so float results are deterministic and within 1 ulp.
@@ -324,6 +380,7 @@ MPGOS_HD MPGOS_INLINE float exp(float x) { return (float)exp((double)x); }
 MPGOS_HD MPGOS_INLINE float log(float x) { return (float)log((double)x); }
 MPGOS_HD MPGOS_INLINE float pow(float x, float y) { return (float)pow((double)x, (double)y); }
 MPGOS_HD MPGOS_INLINE float pow_ctrl(float x, float y) { return (float)pow_ctrl((double)x, (double)y); }
+MPGOS_HD MPGOS_INLINE float ctrl_root(float x, bool fifth) { return (float)ctrl_root((double)x, fifth); }
 MPGOS_HD MPGOS_INLINE float sqrt(float x) { return (float)sqrt((double)x); }   // exact: double has > 2 * 24 + 2 bits
 
 } // namespace math

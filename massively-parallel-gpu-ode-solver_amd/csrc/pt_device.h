@@ -360,7 +360,7 @@ struct Lane {
         }
         // one power for both outcomes: lanes of a wavefront that accept and
         // lanes that reject then share one evaluation
-        CP fac = CP(0.9) * math::pow_ctrl(R, update == 1 ? CP(1.0 / 5.0) : CP(1.0 / 4.0));
+        CP fac = CP(0.9) * math::ctrl_root(R, update == 1);
         if (!math::isfinite(fac)) isfin = 0;
 
         if (isfin == 0) {

@@ -75,6 +75,8 @@ template <int N> D<N> sqrt(const D<N>& a)
 }
 // only the controller calls pow; its result is frozen, so the value suffices
 template <int N> D<N> pow(const D<N>& a, const D<N>& b) { return D<N>(mpgos::math::pow(a.v, b.v)); }
+// the per-thread controller's root, frozen like pow
+template <int N> D<N> ctrl_root(const D<N>& a, bool fifth) { return D<N>(mpgos::math::ctrl_root(a.v, fifth)); }
 template <int N> D<N> frozen(const D<N>& a) { return D<N>(a.v); }
 
 } // namespace dual

@@ -6,7 +6,9 @@
 // elementary functions are charged the operations their source performs
 // (csrc/mpgos_math.h): cos or sin 38 (reduction 5, both polynomial kernels
 // 19 + 14), exp 31, log 28, pow 70 (double-double log 32, product 5,
-// exp 31, final fma 2), sqrt 1.  Used for the
+// exp 31, final fma 2), sqrt 1, the per-thread controller's root 44 (R^(1/5):
+// start polynomials 17, two Newton steps 16, result and correction 11) or 2
+// (R^(1/4) = sqrt(sqrt(R))).  Used for the
 // roofline's algorithmic FLOPs per step.
 #pragma once
 
@@ -48,6 +50,7 @@ F sin(F a);
 F exp(F a);
 F log(F a);
 F pow(F a, F b);
+F ctrl_root(F a, bool fifth);
 F sqrt(F a);
 
 } // namespace flops

@@ -29,6 +29,7 @@ F exp(F a) { g_count += 31; return F(mpgos::math::exp(a.v)); }
 F log(F a) { g_count += 28; return F(mpgos::math::log(a.v)); }
 F pow(F a, F b) { g_count += 70; return F(mpgos::math::pow(a.v, b.v)); }
 F sqrt(F a) { g_count += 1; return F(mpgos::math::sqrt(a.v)); }
+F ctrl_root(F a, bool fifth) { g_count += fifth ? 44 : 2; return F(mpgos::math::ctrl_root(a.v, fifth)); }
 } // namespace flops
 
 template <class Sys, class S>

@@ -90,6 +90,7 @@ struct Instance {
     static bool fin(const S& a) { using std::isfinite; return isfinite(a); }
     static S fm(const S& a, const S& b, const S& c) { using std::fma; return fma(a, b, c); }
     static S pw(const S& a, double e) { using mpgos::math::pow; return pow(a, S(e)); }
+    static S cr(const S& a, bool fifth) { using mpgos::math::ctrl_root; return ctrl_root(a, fifth); }
     static S frz(const S& a) { using oracle::frozen; return frozen(a); }
 
     void read()
@@ -228,7 +229,7 @@ struct Instance {
             ratio = fmn(ratio, tol / err[i]);
         }
         lastRatio = ratio;
-        S factor = S(0.9) * pw(ratio, accept == 1 ? 1.0 / 5.0 : 1.0 / 4.0);
+        S factor = S(0.9) * cr(ratio, accept == 1);   // R^(1/5) or R^(1/4), as the kernel computes it
         if (!fin(factor)) finite = false;
         const bool tiny = dt < S(J.dtmin) * S(1.01);
         if (!finite) {

@@ -20,15 +20,20 @@ pytestmark = pytest.mark.gpu
 
 ROWS = []
 
-# (median, max) bounds: about twice the measured values on the MI355X
+# (median, max) bounds: about twice the measured values on the MI355X.  For
+# plain FP32 on the chaotic Lorenz ensemble the maximum is the distance between
+# two unrelated points of the attractor over the size of the FP64 state (it
+# moved from 1.8 to 7.0 when the controller's R^(1/5) changed in its last
+# bits), so that bound is the attractor's scale, not twice a measured value;
+# the FP32 maxima at C2's 1e-9 moved with it (0.014 to 0.044).
 BOUNDS = {
     ("C0", "FLOAT"): (2.5e-3, 3.6), ("C0", "FLOAT_KAHAN"): (2e-5, 0.22),
-    ("C1", "FLOAT"): (1e-6, 3.6), ("C1", "FLOAT_KAHAN"): (7e-8, 0.19),
+    ("C1", "FLOAT"): (1e-6, 50.0), ("C1", "FLOAT_KAHAN"): (7e-8, 0.19),
     ("C2", "FLOAT"): (2e-5, 0.075), ("C2", "FLOAT_KAHAN"): (1e-5, 0.07),
     ("C3rk4", "FLOAT"): (7e-4, 126.0), ("C3rk4", "FLOAT_KAHAN"): (2.1e-6, 0.094),
     ("C3rkck45", "FLOAT"): (2.3e-3, 92.0), ("C3rkck45", "FLOAT_KAHAN"): (3.9e-6, 0.17),
     ("C4", "FLOAT"): (2.7e-5, 0.14), ("C4", "FLOAT_KAHAN"): (1.3e-6, 4.1e-3),
-    ("C2at1e-9", "FLOAT"): (5.7e-5, 2.8e-2), ("C2at1e-9", "FLOAT_KAHAN"): (3.2e-6, 1.1e-3),
+    ("C2at1e-9", "FLOAT"): (5.7e-5, 9e-2), ("C2at1e-9", "FLOAT_KAHAN"): (3.2e-6, 1.1e-3),
 }
 
 

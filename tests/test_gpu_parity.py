@@ -269,19 +269,20 @@ def test_device_math_matches_host_bitwise(gpu):
     try:
         L = gpu.lib()
         for name, fn in FN.items():
-            arg = np.abs(x) if name in ("log", "pow", "pow_ctrl", "sqrt") else (np.clip(x, -700, 700) if name == "exp" else x)
+            arg = np.abs(x) if name in ("log", "pow", "pow_ctrl", "ctrl_root", "sqrt") else (np.clip(x, -700, 700) if name == "exp" else x)
             if name in ("cos", "sin"):      # large arguments: up to 2^52 and the NaN range beyond
                 big = 10.0 ** rng.uniform(4, 17, 2000) * np.where(rng.uniform(size=2000) < 0.5, -1.0, 1.0)
                 arg = np.concatenate([arg[:-2000], big])
-            if name in ("cos", "sin", "exp", "log", "pow", "pow_ctrl", "sqrt"):     # special arguments at the end
+            if name in ("cos", "sin", "exp", "log", "pow", "pow_ctrl", "ctrl_root", "sqrt"):     # special arguments at the end
                 arg = np.concatenate([arg[:-len(SPECIAL)], SPECIAL])
             arg = np.ascontiguousarray(arg)
             assert hip.hipMemcpy(bufs[0], arg.ctypes.data, nbytes, 1) == 0
-            assert hip.hipMemcpy(bufs[1], y.ctypes.data, nbytes, 1) == 0
+            yy = (np.arange(arg.size) % 2).astype(np.float64) if name == "ctrl_root" else y   # fifth or fourth root
+            assert hip.hipMemcpy(bufs[1], yy.ctypes.data, nbytes, 1) == 0
             assert L.mpgos_math_eval_device(fn, bufs[0], bufs[1], bufs[2], arg.size) == 0
             dev = np.empty_like(arg)
             assert hip.hipMemcpy(dev.ctypes.data, bufs[2], nbytes, 2) == 0
-            host = host_eval(name, arg, y if name in ("pow", "pow_ctrl") else None)
+            host = host_eval(name, arg, yy if name in ("pow", "pow_ctrl", "ctrl_root") else None)
             both_nan = np.isnan(dev) & np.isnan(host)
             assert np.array_equal(dev.view(np.uint64)[~both_nan], host.view(np.uint64)[~both_nan]), name
             assert np.array_equal(np.isnan(dev), np.isnan(host)), name

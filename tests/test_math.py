@@ -13,7 +13,7 @@ import pytest
 
 from mpgos import lib
 
-FN = {"cos": 0, "sin": 1, "exp": 2, "log": 3, "pow": 4, "sqrt": 5, "pow_ctrl": 6}
+FN = {"cos": 0, "sin": 1, "exp": 2, "log": 3, "pow": 4, "sqrt": 5, "pow_ctrl": 6, "ctrl_root": 7}
 
 
 def host_eval(name, x, y=None):
@@ -201,3 +201,22 @@ def test_controller_power_equals_pow_on_its_domain():
         nan = np.isnan(a) & np.isnan(b)
         assert np.array_equal(np.isnan(a), np.isnan(b))
         assert np.array_equal(a.view(np.uint64)[~nan], b.view(np.uint64)[~nan])
+
+
+def test_controller_root_against_mpmath():
+    """ctrl_root (the per-thread controller's R^(1/5) after an accepted step,
+    R^(1/4) after a rejection, from a start polynomial, two Newton steps and
+    square roots) against mpmath over the whole non-negative range, subnormals
+    and the controller's fixed point 0.9^-5 included: within 2 ulp for the
+    fifth root, 1 ulp for the fourth; 0, +inf and NaN pass through."""
+    x = np.concatenate([10.0 ** rng.uniform(-323, 308, 4000), rng.uniform(0.0, 2.0, 4000), rng.uniform(0.9, 1.1, 2000),
+                        [5e-324, 2.2250738585072014e-308, 2.0 ** -1000, 2.0 ** -1001, 1.0, 32.0, 0.9 ** -5, 1e30,
+                         1.7976931348623157e308]])
+    for fifth, e in ((1.0, mpmath.mpf(1) / 5), (0.0, mpmath.mpf(1) / 4)):
+        got = host_eval("ctrl_root", x, fifth)
+        ref = mp_ref(lambda v: v ** e, x)
+        assert ulp_err(got, ref).max() <= (2.0 if fifth else 1.0), (fifth, ulp_err(got, ref).max())
+        special = host_eval("ctrl_root", np.array([0.0, np.inf, np.nan]), fifth)
+        assert special[0] == 0.0 and special[1] == np.inf and np.isnan(special[2])
+    # the fixed point of h <- 0.9 h (tol/err)^(1/5) stays a fixed point
+    assert abs(0.9 * host_eval("ctrl_root", np.array([0.9 ** -5]), 1.0)[0] - 1.0) <= 2.3e-16
