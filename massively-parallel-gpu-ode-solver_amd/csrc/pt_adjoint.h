@@ -19,7 +19,7 @@
 //      (controller, rejections and all) -- identical arithmetic, so the same
 //      accepted steps -- storing (x_n, t_n, h_n) of each accepted step in a
 //      per-lane segment buffer, in LDS when K segment records of the
-//      workgroup fit there (lds[(j*(SD+2) + c)*blockDim + lane]: the records
+//      workgroup fit there (per wavefront: lds[(j*(SD+2) + c)*64 + lane]: the records
 //      never leave the compute unit), else in HBM seg[((j*(SD+2)) + c)*NT + i],
 //      j < K;
 //   2. walk the segment backwards (while the next segment down is re-stepped
@@ -264,13 +264,17 @@ __device__ __forceinline__ void reverse_body(const AdjArgs<Sys>& A)
     const int tid = threadIdx.x + blockIdx.x * blockDim.x;
     if (tid >= a.nActive) return;
     const int NT = a.NT;
-    // segment record word w of this lane
+    // segment record word w of this lane: in LDS one region per wavefront,
+    // word-major with a lane stride of 64 doubles (a record's words are then
+    // constant offsets from one address); in HBM word-major over all lanes
     extern __shared__ __align__(16) unsigned char mpgos_dyn_lds[];
-    double* const segBase = LDS_SEG ? reinterpret_cast<double*>(mpgos_dyn_lds + adjoint_shared_bytes<Sys>(a.preferShared))
-                                    : A.seg;
+    double* const segBase =
+        LDS_SEG ? reinterpret_cast<double*>(mpgos_dyn_lds + adjoint_shared_bytes<Sys>(a.preferShared)) +
+                      (size_t)(threadIdx.x >> 6) * ((size_t)A.K * (SD + 2) * 64) + (threadIdx.x & 63)
+                : A.seg + tid;
     auto seg = [&](size_t w) -> double& {
-        if constexpr (LDS_SEG) return segBase[w * blockDim.x + threadIdx.x];
-        else return segBase[w * NT + tid];
+        if constexpr (LDS_SEG) return segBase[w * 64];
+        else return segBase[w * NT];
     };
     const long long total = a.nacc[tid];
     double lam[SD], mu[Pos<NCP>::value];

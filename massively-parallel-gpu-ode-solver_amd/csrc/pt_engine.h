@@ -928,9 +928,9 @@ public:
     }
 
     // Segment records of the reverse pass in LDS: K records of SD + 2 doubles
-    // per lane of the workgroup, when they fit in 64 KiB with the staged shared
-    // parameters.
-    size_t segment_lds_bytes(int K) const { return (size_t)K * (SD + 2) * sizeof(double) * (size_t)blockSize; }
+    // per lane of the workgroup (lanes counted in whole wavefronts), when they
+    // fit in 64 KiB with the staged shared parameters.
+    size_t segment_lds_bytes(int K) const { return (size_t)K * (SD + 2) * sizeof(double) * (size_t)((blockSize + 63) / 64 * 64); }
     bool segment_in_lds(int K) const
     {
         return adjoint_shared_bytes<Sys>(preferShared) + segment_lds_bytes(K) <= (size_t)64 * 1024;
