@@ -93,6 +93,7 @@ struct Args {
     P* DOx;                   // unit index + component * NU + k * NU * UD
     // global scope
     P* gPAR; int* gPARi; int* cIndex; P* cMat;
+    const P* cTile;           // dense-contraction form: the matrix in column-block pairs per lane (null: read cMat)
     long long* nacc; long long* nrej; int* status;
     int NS, UPS, SPB, nActive, NDO, CBW, CCI, cMatSize, sharedGlobals, sharedMatrix;
     int doFreq;
@@ -255,7 +256,8 @@ struct Unit {
 // operations, so the wait for a group's B operands from LDS would wait for all
 // of its M loads too); otherwise M is in LDS.
 template <int NC, bool GLOBAL_M = true, int G = 8>
-__device__ __forceinline__ void mfma_contract44(const double* M, const double* cptB, double* coup, int UPS, int SPB)
+__device__ __forceinline__ void mfma_contract44(const double* M, const double* cptB, double* coup, int UPS, int SPB,
+                                                const double* Mt = nullptr)
 {
     using GPtr = const __attribute__((address_space(1))) double*;
     const int l = threadIdx.x & 63;
@@ -285,16 +287,56 @@ __device__ __forceinline__ void mfma_contract44(const double* M, const double* c
                 }
             };
             double acc0 = 0.0, acc1 = 0.0;              // systems 0..3, 4..7
-            for (int c0 = 0; c0 < blocks; c0 += G) {
-                double a0[G];
+            if (GLOBAL_M && Mt) {
+                // pair-tiled copy (tile_pair_matrix): one 16-byte load per lane
+                // brings the lane's A values of two column blocks, half the
+                // vector-memory instructions of the column-major reads
+                struct alignas(16) Pair { double x, y; };
+                using GP = const __attribute__((address_space(1))) Pair*;
+                const Pair* At = reinterpret_cast<const Pair*>(Mt + ((size_t)i * tiles + t) * (size_t)blocks * 64) + l;
+                for (int c0 = 0; c0 < blocks; c0 += G) {
+                    double a0[G];
 #pragma unroll
-                for (int d = 0; d < G; d++) a0[d] = ldA(c0 + d);
-                step(a0, c0, acc0, acc1);
+                    for (int d = 0; d < G; d += 2) {
+                        const GP q = (GP)(At + (size_t)((c0 + d) >> 1) * 64);
+                        a0[d] = q->x;
+                        a0[d + 1] = q->y;
+                    }
+                    step(a0, c0, acc0, acc1);
+                }
+            } else {
+                for (int c0 = 0; c0 < blocks; c0 += G) {
+                    double a0[G];
+#pragma unroll
+                    for (int d = 0; d < G; d++) a0[d] = ldA(c0 + d);
+                    step(a0, c0, acc0, acc1);
+                }
             }
             if (sys < SPB) coup[mfma_index(i, sys, t * 16 + drow, UPS)] = acc0;
             if (sys + 4 < SPB) coup[mfma_index(i, sys + 4, t * 16 + drow, UPS)] = acc1;
         }
     }
+}
+
+// The pair-tiled copy of the dense coupling matrices read by mfma_contract44:
+// for matrix i, 16-unit tile t, column-block pair cp, lane l and j in {0, 1},
+//   Mt[(((i * tiles + t) * blocks / 2 + cp) * 64 + l) * 2 + j]
+//     = M_i[t * 16 + l % 16 + (4 * (2 cp + j) + l / 16) * UPS],
+// i.e. the A operand of lane l for column blocks 2 cp and 2 cp + 1 side by
+// side (the same values, so the contraction's results are unchanged).
+template <int Unused = 0>
+__global__ void tile_pair_matrix(const double* M, double* Mt, int UPS, int NC)
+{
+    const size_t n = (size_t)NC * UPS * UPS;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int tiles = UPS >> 4, pairs = UPS >> 3;
+    const int j = (int)(k & 1), l = (int)((k >> 1) & 63);
+    size_t r = k >> 7;
+    const int cp = (int)(r % pairs); r /= pairs;
+    const int t = (int)(r % tiles);
+    const int i = (int)(r / tiles);
+    Mt[k] = M[(size_t)i * UPS * UPS + t * 16 + (l & 15) + (size_t)(4 * (2 * cp + j) + (l >> 4)) * UPS];
 }
 
 // One workgroup = SPB systems.  A system has UPS / PAIRS lanes; lane t belongs
@@ -469,7 +511,7 @@ __device__ void pb_body(const Args<Sys, P, typename SysScope<Sys, P, KAHAN>::CP>
             }
             __syncthreads();
             if (a.sharedMatrix) mfma_contract44<NC, false>(M, cptBuf, coupBuf, UPS, SPB);
-            else mfma_contract44<NC, true>(M, cptBuf, coupBuf, UPS, SPB);
+            else mfma_contract44<NC, true>(M, cptBuf, coupBuf, UPS, SPB, (const double*)a.cTile);
             __syncthreads();
             if (run) {
 #pragma unroll

@@ -86,6 +86,7 @@ public:
     Buffer<P> lamB, gUB, gSB;           // dL/dx, dL/d(unit parameters), dL/d(system parameters)
     double* recX = nullptr; double* recT = nullptr; double* recH = nullptr; double* seg = nullptr; double* cMatT = nullptr;
     double* stg = nullptr; double* gSp = nullptr;
+    double* cTile = nullptr;                // pair-tiled copy of the dense coupling matrices (dense-contraction form)
     size_t recXn = 0, recTn = 0, recHn = 0, segn = 0, cMatTn = 0, stgn = 0, gSpn = 0;
     int adjK = 0, adjCap = 0, adjStepCap = 0;
 
@@ -93,7 +94,7 @@ public:
     {
         if (strm) (void)hipStreamSynchronize(strm);
         lamB.release(); gUB.release(); gSB.release();
-        for (double* q : {recX, recT, recH, seg, cMatT, stg, gSp})
+        for (double* q : {recX, recT, recH, seg, cMatT, stg, gSp, cTile})
             if (q) (void)hipFree(q);
         for (auto* b : {&X, &uPAR, &uACC, &TD, &T, &sPAR, &sACC, &strength, &DOt, &DOx, &gPAR, &cMat}) b->release();
         for (auto* b : {&uACCi, &sACCi, &DOidx, &gPARi, &cIndex, &status}) b->release();
@@ -473,6 +474,19 @@ public:
                 else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, kMaxBlock, true>), dim3(grid), dim3(threads), lds, strm, a);
             }
         } else {
+            if (cpl == kMfmaCoupling && !a.sharedMatrix && std::is_same<P, double>::value) {
+                // the matrices in column-block pairs per lane, refreshed every Solve
+                // (a kernel on the solver's stream, so graph capture records it)
+                const size_t n = (size_t)NC * UPS * UPS;
+                if (!cTile && hipMalloc((void**)&cTile, n * sizeof(double)) != hipSuccess) {
+                    cTile = nullptr;
+                    set_error("per-block: allocation of the tiled coupling matrix failed");
+                    return MPGOS_E_MEMORY;
+                }
+                hipLaunchKernelGGL(tile_pair_matrix<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, strm,
+                                   (const double*)cMat.d, cTile, UPS, NC);
+                a.cTile = (const P*)cTile;
+            }
             if (cpl == kMfmaCoupling && mfmaPairs == 2) {
                 if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RK4, 2>), dim3(grid), dim3(threads), lds, strm, a);
                 else hipLaunchKernelGGL((pb_mfma_kernel<Sys, P, MPGOS_RKCK45, 2>), dim3(grid), dim3(threads), lds, strm, a);
