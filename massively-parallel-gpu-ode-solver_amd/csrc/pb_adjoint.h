@@ -49,6 +49,7 @@ struct AdjArgs {
     double* gU;           // out: dL/d(unit parameters)             [NUP][NU]
     double* gS;           // out: dL/d(system parameters)           [NSP][NS]
     const double* cMatT;  // transposed coupling matrices in the stored form of cMat
+    const double* cTileT; // their pair-tiled copy (dense-contraction form; null: read cMatT)
 };
 
 // sum over the matrix row u of M_i (any storage form) against vals[col * NCp + i]
@@ -623,7 +624,7 @@ __device__ void pb_replay_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sy
             }
         }
         lds_barrier();
-        mfma_contract44<NC>(a.cMat, cptB, coup, UPS, SPB);
+        mfma_contract44<NC>(a.cMat, cptB, coup, UPS, SPB, a.cTile);
         lds_barrier();
         if (run) {
 #pragma unroll
@@ -824,7 +825,7 @@ __device__ void pb_sweep_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys
                 }
             }
             __syncthreads();
-            mfma_contract44<NC>(A.cMatT, cptB, coup, UPS, SPB);
+            mfma_contract44<NC>(A.cMatT, cptB, coup, UPS, SPB, A.cTileT);
             __syncthreads();
 #pragma unroll
             for (int p = 0; p < PAIRS; p++) {

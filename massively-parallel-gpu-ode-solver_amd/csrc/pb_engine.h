@@ -86,7 +86,9 @@ public:
     Buffer<P> lamB, gUB, gSB;           // dL/dx, dL/d(unit parameters), dL/d(system parameters)
     double* recX = nullptr; double* recT = nullptr; double* recH = nullptr; double* seg = nullptr; double* cMatT = nullptr;
     double* stg = nullptr; double* gSp = nullptr;
-    double* cTile = nullptr;                // pair-tiled copy of the dense coupling matrices (dense-contraction form)
+    double* cTile = nullptr;                // pair-tiled copies (tile_pair_matrix) of the dense coupling matrices
+    double* cTileT = nullptr;               // and of their transposes (adjoint sweep)
+    size_t cTileN = 0, cTileTN = 0;
     size_t recXn = 0, recTn = 0, recHn = 0, segn = 0, cMatTn = 0, stgn = 0, gSpn = 0;
     int adjK = 0, adjCap = 0, adjStepCap = 0;
 
@@ -94,7 +96,7 @@ public:
     {
         if (strm) (void)hipStreamSynchronize(strm);
         lamB.release(); gUB.release(); gSB.release();
-        for (double* q : {recX, recT, recH, seg, cMatT, stg, gSp, cTile})
+        for (double* q : {recX, recT, recH, seg, cMatT, stg, gSp, cTile, cTileT})
             if (q) (void)hipFree(q);
         for (auto* b : {&X, &uPAR, &uACC, &TD, &T, &sPAR, &sACC, &strength, &DOt, &DOx, &gPAR, &cMat}) b->release();
         for (auto* b : {&uACCi, &sACCi, &DOidx, &gPARi, &cIndex, &status}) b->release();
@@ -477,14 +479,7 @@ public:
             if (cpl == kMfmaCoupling && !a.sharedMatrix && std::is_same<P, double>::value) {
                 // the matrices in column-block pairs per lane, refreshed every Solve
                 // (a kernel on the solver's stream, so graph capture records it)
-                const size_t n = (size_t)NC * UPS * UPS;
-                if (!cTile && hipMalloc((void**)&cTile, n * sizeof(double)) != hipSuccess) {
-                    cTile = nullptr;
-                    set_error("per-block: allocation of the tiled coupling matrix failed");
-                    return MPGOS_E_MEMORY;
-                }
-                hipLaunchKernelGGL(tile_pair_matrix<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, strm,
-                                   (const double*)cMat.d, cTile, UPS, NC);
+                if (int rc = tile_pairs((const double*)cMat.d, cTile, cTileN)) return rc;
                 a.cTile = (const P*)cTile;
             }
             if (cpl == kMfmaCoupling && mfmaPairs == 2) {
@@ -503,6 +498,27 @@ public:
         }
         MPGOS_HIP_TRY(hipGetLastError());
         MPGOS_HIP_TRY(hipEventRecord(evStop, strm));
+        return MPGOS_OK;
+    }
+
+    // The pair-tiled copy of NC full UPS x UPS matrices at src (column-major),
+    // written by a kernel on the solver's stream (allocated on first use; the
+    // size never changes for a solver, so no buffer in use is freed).
+    int tile_pairs(const double* src, double*& dst, size_t& have)
+    {
+        const size_t n = (size_t)NC * UPS * UPS;
+        if (n > have) {
+            if (dst) { MPGOS_HIP_TRY(hipStreamSynchronize(strm)); (void)hipFree(dst); }
+            dst = nullptr; have = 0;
+            if (hipMalloc((void**)&dst, n * sizeof(double)) != hipSuccess) {
+                dst = nullptr;
+                set_error("per-block: allocation of the tiled coupling matrix failed");
+                return MPGOS_E_MEMORY;
+            }
+            have = n;
+        }
+        hipLaunchKernelGGL(tile_pair_matrix<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, strm, src, dst, UPS, NC);
+        MPGOS_HIP_TRY(hipGetLastError());
         return MPGOS_OK;
     }
 
@@ -548,6 +564,10 @@ public:
             const int threads = cpl == kMfmaCoupling ? kMfmaSystems * UPS / 4 : ((SPB * UPS + 63) / 64) * 64;
             const int grid = NS / spb + (NS % spb == 0 ? 0 : 1);
             MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+            if (cpl == kMfmaCoupling && !a.sharedMatrix) {
+                if ((rc = tile_pairs((const double*)cMat.d, cTile, cTileN))) return rc;
+                a.cTile = cTile;
+            }
             if (cpl == kMfmaCoupling) {
                 if (algorithm == MPGOS_RK4) hipLaunchKernelGGL((pb_record_mfma_kernel<Sys, MPGOS_RK4>), dim3(grid), dim3(threads), lds, strm, a);
                 else hipLaunchKernelGGL((pb_record_mfma_kernel<Sys, MPGOS_RKCK45>), dim3(grid), dim3(threads), lds, strm, a);
@@ -603,6 +623,7 @@ public:
             a.recX = recX; a.recT = recT; a.recH = recH; a.recK = adjK; a.recCkCap = adjCap; a.recStepCap = adjStepCap;
             AdjArgs<Sys> A;
             A.seg = seg; A.stg = stg; A.gSp = nullptr; A.lam = lamB.d; A.gU = gUB.d; A.gS = gSB.d; A.cMatT = cMatT;
+            A.cTileT = nullptr;
             lastForm = cpl;
             if (cpl == kMfmaCoupling) {
                 // one replay and one sweep launch per round; rounds = the largest
@@ -626,6 +647,9 @@ public:
                 const int grid = NS / kMfmaSystems + (NS % kMfmaSystems == 0 ? 0 : 1);
                 const int fgrid = (int)((NU + 255) / 256);
                 MPGOS_HIP_TRY(hipEventRecord(evStart, strm));
+                if ((rc = tile_pairs((const double*)cMat.d, cTile, cTileN)) || (rc = tile_pairs(cMatT, cTileT, cTileTN))) return rc;
+                a.cTile = cTile;
+                A.cTileT = cTileT;
                 for (int r = 0; r < rounds; r++) {
                     if (algorithm == MPGOS_RK4) {
                         hipLaunchKernelGGL((pb_replay_mfma_kernel<Sys, MPGOS_RK4, kMfmaPairs>), dim3(grid), dim3(threads), lds, strm, a, A, r);
