@@ -624,7 +624,7 @@ __device__ void pb_replay_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sy
             }
         }
         lds_barrier();
-        mfma_contract44<NC>(a.cMat, cptB, coup, UPS, SPB, a.cTile);
+        mfma_contract44<NC, true, 8, mfma_tiles_together<Sys, PAIRS>()>(a.cMat, cptB, coup, UPS, SPB, a.cTile);
         lds_barrier();
         if (run) {
 #pragma unroll
@@ -825,7 +825,7 @@ __device__ void pb_sweep_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys
                 }
             }
             __syncthreads();
-            mfma_contract44<NC>(A.cMatT, cptB, coup, UPS, SPB, A.cTileT);
+            mfma_contract44<NC, true, 8, mfma_tiles_together<Sys, PAIRS>()>(A.cMatT, cptB, coup, UPS, SPB, A.cTileT);
             __syncthreads();
 #pragma unroll
             for (int p = 0; p < PAIRS; p++) {

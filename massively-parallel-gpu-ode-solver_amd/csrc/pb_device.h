@@ -67,6 +67,16 @@ constexpr int kMaxBlock = 1024;
 constexpr int kMfmaSystems = 8;          // systems per workgroup of the dense-contraction form
 constexpr int kMfmaPairs = 4;            // default pairs per lane of the dense-contraction form (2 * UPS threads)
 
+// Tiles of the dense-contraction form that a wave contracts together: all of
+// its PAIRS / 2 tiles when the lane's unit states and stage vectors leave room
+// in the registers (UD * PAIRS <= 8), else one at a time.  Only the first way
+// keeps leading matrix pairs in LDS (the results then reuse the rows of the
+// coupling terms, which frees the LDS for them).
+template <class Sys, int PAIRS>
+__host__ __device__ constexpr int mfma_tiles_together() { return PAIRS >= 4 && Sys::UD * PAIRS <= 8 ? PAIRS / 2 : 1; }
+template <class Sys, int PAIRS>
+__host__ __device__ constexpr bool mfma_lds_matrix() { return mfma_tiles_together<Sys, PAIRS>() > 1; }
+
 // LDS of the dense-contraction form: the B operand (coupling terms) and the
 // contraction results, one row of UPS + 4 doubles per (coupling, block
 // system).  A wave's writes and reads of one system's consecutive units fall
@@ -94,6 +104,7 @@ struct Args {
     // global scope
     P* gPAR; int* gPARi; int* cIndex; P* cMat;
     const P* cTile;           // dense-contraction form: the matrix in column-block pairs per lane (null: read cMat)
+    int ldsPairs;             // dense-contraction form: leading column-block pairs of every tile of cTile held in LDS
     long long* nacc; long long* nrej; int* status;
     int NS, UPS, SPB, nActive, NDO, CBW, CCI, cMatSize, sharedGlobals, sharedMatrix;
     int doFreq;
@@ -150,12 +161,12 @@ template <> __device__ __forceinline__ float dmax<float>(float a, float b) { ret
 // buffer plus a [coupling][unit][system] result buffer in the dense form),
 // optional global parameters and coupling matrix copies.
 struct LdsLayout {
-    size_t cpt, coup, g, gi, m, total;
+    size_t cpt, coup, g, gi, m, mt, total;
 };
 
 template <class Sys, class P, bool KAHAN = false>
 __host__ __device__ inline LdsLayout lds_layout(int cpl, int SPB, int UPS, int sharedGlobals, int sharedMatrix,
-                                                int cMatSize)
+                                                int cMatSize, int ldsPairs = 0)
 {
     constexpr int NCp = Pos<Sys::NC>::value;
     LdsLayout L;
@@ -165,7 +176,10 @@ __host__ __device__ inline LdsLayout lds_layout(int cpl, int SPB, int UPS, int s
     if (cpl == kMfmaCoupling) {
         o += (size_t)NCp * kMfmaSystems * mfma_row(UPS) * sizeof(P);
         L.coup = o;
-        o += (size_t)NCp * kMfmaSystems * mfma_row(UPS) * sizeof(P);
+        // with matrix pairs in LDS the contraction results overwrite the
+        // coupling terms once every wave has finished reading them
+        if (ldsPairs > 0) L.coup = L.cpt;
+        else o += (size_t)NCp * kMfmaSystems * mfma_row(UPS) * sizeof(P);
     } else {
         o += 2 * (size_t)SPB * UPS * NCp * sizeof(P);
         L.coup = o;
@@ -177,6 +191,9 @@ __host__ __device__ inline LdsLayout lds_layout(int cpl, int SPB, int UPS, int s
     if (sharedMatrix) o += (size_t)Sys::NC * cMatSize * sizeof(P);
     L.gi = o;
     if (sharedGlobals) o += (size_t)Sys::NiGP * sizeof(int);
+    o = (o + 15) & ~(size_t)15;
+    L.mt = o;
+    if (cpl == kMfmaCoupling) o += (size_t)NCp * (UPS / 16) * ldsPairs * 64 * 16;
     L.total = o;
     return L;
 }
@@ -255,45 +272,129 @@ struct Unit {
 // global loads (a generic pointer gives flat loads, which also count as LDS
 // operations, so the wait for a group's B operands from LDS would wait for all
 // of its M loads too); otherwise M is in LDS.
-template <int NC, bool GLOBAL_M = true, int G = 8>
+template <int NC, bool GLOBAL_M = true, int G = 8, int TPW = 1>
 __device__ __forceinline__ void mfma_contract44(const double* M, const double* cptB, double* coup, int UPS, int SPB,
-                                                const double* Mt = nullptr)
+                                                const double* Mt = nullptr, const double* Ml = nullptr, int ldsPairs = 0)
 {
     using GPtr = const __attribute__((address_space(1))) double*;
+    struct alignas(16) Pair { double x, y; };
+    using GP = const __attribute__((address_space(1))) Pair*;
+    using LP = const __attribute__((address_space(3))) Pair*;
     const int l = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
     const int nw = blockDim.x >> 6;
     const int tiles = UPS >> 4;
     const int r16 = l & 15, k4 = l >> 4, sys = l & 3;
     const int drow = 4 * ((l >> 2) & 3) + (l >> 4);
     const int blocks = UPS >> 2;                        // a multiple of G (UPS is a multiple of 32)
     const size_t colStride = (size_t)4 * UPS;
-    // one 16-unit tile per pass; its two system halves are two independent chains
-    for (int t = threadIdx.x >> 6; t < tiles; t += nw) {
+    const int RS = mfma_row(UPS);
+    auto store = [&](int i, int t, double acc0, double acc1) {
+        if (sys < SPB) coup[mfma_index(i, sys, t * 16 + drow, UPS)] = acc0;
+        if (sys + 4 < SPB) coup[mfma_index(i, sys + 4, t * 16 + drow, UPS)] = acc1;
+    };
+    // pair-tiled copy (tile_pair_matrix): one 16-byte load per lane brings the
+    // lane's A values of two column blocks, half the vector-memory
+    // instructions of column-major reads.  Per tile, row and system the MFMAs
+    // run in column order, as in the column-major loop at the end.
+    if (GLOBAL_M && Mt && TPW > 1 && tiles == TPW * nw) {
+        // every tile of the wave (w, w + nw, ...) at once, G / TPW column
+        // blocks per tile and group: one B read per column block and chain
+        // serves all of them, and the 2 TPW accumulator chains are
+        // independent.  The leading ldsPairs pairs of every tile (a multiple
+        // of G / 2) come from the workgroup's LDS copy (Ml, same layout),
+        // which takes their loads off the vector-memory path; the global
+        // groups are read two buffers deep, the first during the LDS part.
+        // coup == cptB: the results overwrite the B operand, so they are
+        // stored after every wave of the workgroup has finished.
+        constexpr int TT = TPW > 1 ? TPW : 2;
+        constexpr int GB = G / TT;                      // column blocks per tile and group
+        static_assert(GB % 2 == 0, "whole pairs per group");
+        double res[Pos<NC>::value][TT][2];
 #pragma unroll
         for (int i = 0; i < NC; i++) {
-            const double* Ai = M + (size_t)i * UPS * UPS + t * 16 + r16 + (size_t)k4 * UPS;
-            const int RS = mfma_row(UPS);
             const double* Bi = cptB + mfma_index(i, sys, k4, UPS);
-            auto ldA = [&](int blk) -> double {
-                if constexpr (GLOBAL_M) return *(GPtr)(Ai + (size_t)blk * colStride);
-                else return Ai[(size_t)blk * colStride];
-            };
-            auto step = [&](const double* a, int c0, double& acc0, double& acc1) {
+            double (&acc)[TT][2] = res[i];
 #pragma unroll
-                for (int d = 0; d < G; d++) {
+            for (int k = 0; k < TT; k++) acc[k][0] = acc[k][1] = 0.0;
+            auto group = [&](const double (*a)[GB], int c0) {
+#pragma unroll
+                for (int d = 0; d < GB; d++) {
                     const double b0 = Bi[(c0 + d) * 4], b1 = Bi[(c0 + d) * 4 + 4 * RS];
-                    acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b0, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(a[d], b1, acc1, 0, 0, 0);
+#pragma unroll
+                    for (int k = 0; k < TT; k++) {
+                        acc[k][0] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[k][d], b0, acc[k][0], 0, 0, 0);
+                        acc[k][1] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[k][d], b1, acc[k][1], 0, 0, 0);
+                    }
                 }
             };
-            double acc0 = 0.0, acc1 = 0.0;              // systems 0..3, 4..7
-            if (GLOBAL_M && Mt) {
-                // pair-tiled copy (tile_pair_matrix): one 16-byte load per lane
-                // brings the lane's A values of two column blocks, half the
-                // vector-memory instructions of the column-major reads
-                struct alignas(16) Pair { double x, y; };
-                using GP = const __attribute__((address_space(1))) Pair*;
+            const Pair* At = reinterpret_cast<const Pair*>(Mt) + ((size_t)i * tiles + w) * (size_t)blocks * 32 + l;
+            const size_t gs = (size_t)nw * blocks * 32;          // the wave's next tile
+            auto load = [&](double (*a)[GB], int c0) {
+#pragma unroll
+                for (int k = 0; k < TT; k++)
+#pragma unroll
+                    for (int d = 0; d < GB; d += 2) {
+                        const GP q = (GP)(At + k * gs + (size_t)((c0 + d) >> 1) * 64);
+                        a[k][d] = q->x;
+                        a[k][d + 1] = q->y;
+                    }
+            };
+            const int cg = Ml ? 2 * ldsPairs : 0;
+            const int n = (blocks - cg) / GB;
+            double a0[TT][GB], a1[TT][GB];
+            if (n > 0) load(a0, cg);
+            if (Ml) {
+                const Pair* Lt = reinterpret_cast<const Pair*>(Ml) + ((size_t)i * tiles + w) * (size_t)ldsPairs * 64 + l;
+                const size_t ls = (size_t)nw * ldsPairs * 64;
+                for (int cp = 0; cp < ldsPairs; cp += GB / 2) {
+                    double a[TT][GB];
+#pragma unroll
+                    for (int k = 0; k < TT; k++)
+#pragma unroll
+                        for (int d = 0; d < GB; d += 2) {
+                            const LP q = (LP)(Lt + k * ls + (size_t)(cp + (d >> 1)) * 64);
+                            a[k][d] = q->x;
+                            a[k][d + 1] = q->y;
+                        }
+                    group(a, 2 * cp);
+                }
+            }
+            if (n > 0) {
+                // every load in the loop body is unconditional, so the wait
+                // before a group's MFMAs counts only that group's loads
+                int g = 0;
+                for (; g + 2 < n; g += 2) {
+                    load(a1, cg + (g + 1) * GB);
+                    group(a0, cg + g * GB);
+                    load(a0, cg + (g + 2) * GB);
+                    group(a1, cg + (g + 1) * GB);
+                }
+                if (g + 1 < n) {
+                    load(a1, cg + (g + 1) * GB);
+                    group(a0, cg + g * GB);
+                    group(a1, cg + (g + 1) * GB);
+                } else {
+                    group(a0, cg + g * GB);
+                }
+            }
+        }
+        if (coup == cptB) lds_barrier();
+#pragma unroll
+        for (int i = 0; i < NC; i++)
+#pragma unroll
+            for (int k = 0; k < TT; k++) store(i, w + k * nw, res[i][k][0], res[i][k][1]);
+        return;
+    }
+    if (GLOBAL_M && Mt) {
+        // one tile at a time, G column blocks per group (coup and cptB are
+        // distinct here)
+        for (int t = w; t < tiles; t += nw) {
+#pragma unroll
+            for (int i = 0; i < NC; i++) {
+                const double* Bi = cptB + mfma_index(i, sys, k4, UPS);
                 const Pair* At = reinterpret_cast<const Pair*>(Mt + ((size_t)i * tiles + t) * (size_t)blocks * 64) + l;
+                double acc0 = 0.0, acc1 = 0.0;          // systems 0..3, 4..7
                 for (int c0 = 0; c0 < blocks; c0 += G) {
                     double a0[G];
 #pragma unroll
@@ -302,18 +403,41 @@ __device__ __forceinline__ void mfma_contract44(const double* M, const double* c
                         a0[d] = q->x;
                         a0[d + 1] = q->y;
                     }
-                    step(a0, c0, acc0, acc1);
-                }
-            } else {
-                for (int c0 = 0; c0 < blocks; c0 += G) {
-                    double a0[G];
 #pragma unroll
-                    for (int d = 0; d < G; d++) a0[d] = ldA(c0 + d);
-                    step(a0, c0, acc0, acc1);
+                    for (int d = 0; d < G; d++) {
+                        const double b0 = Bi[(c0 + d) * 4], b1 = Bi[(c0 + d) * 4 + 4 * RS];
+                        acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(a0[d], b0, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(a0[d], b1, acc1, 0, 0, 0);
+                    }
+                }
+                store(i, t, acc0, acc1);
+            }
+        }
+        return;
+    }
+    // one 16-unit tile per pass; its two system halves are two independent chains
+    for (int t = w; t < tiles; t += nw) {
+#pragma unroll
+        for (int i = 0; i < NC; i++) {
+            const double* Ai = M + (size_t)i * UPS * UPS + t * 16 + r16 + (size_t)k4 * UPS;
+            const double* Bi = cptB + mfma_index(i, sys, k4, UPS);
+            auto ldA = [&](int blk) -> double {
+                if constexpr (GLOBAL_M) return *(GPtr)(Ai + (size_t)blk * colStride);
+                else return Ai[(size_t)blk * colStride];
+            };
+            double acc0 = 0.0, acc1 = 0.0;              // systems 0..3, 4..7
+            for (int c0 = 0; c0 < blocks; c0 += G) {
+                double a0[G];
+#pragma unroll
+                for (int d = 0; d < G; d++) a0[d] = ldA(c0 + d);
+#pragma unroll
+                for (int d = 0; d < G; d++) {
+                    const double b0 = Bi[(c0 + d) * 4], b1 = Bi[(c0 + d) * 4 + 4 * RS];
+                    acc0 = __builtin_amdgcn_mfma_f64_4x4x4f64(a0[d], b0, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_4x4x4f64(a0[d], b1, acc1, 0, 0, 0);
                 }
             }
-            if (sys < SPB) coup[mfma_index(i, sys, t * 16 + drow, UPS)] = acc0;
-            if (sys + 4 < SPB) coup[mfma_index(i, sys + 4, t * 16 + drow, UPS)] = acc1;
+            store(i, t, acc0, acc1);
         }
     }
 }
@@ -362,7 +486,8 @@ __device__ void pb_body(const Args<Sys, P, typename SysScope<Sys, P, KAHAN>::CP>
     using math::fma;
 
     extern __shared__ __align__(16) unsigned char mpgos_pb_lds[];
-    const LdsLayout L = lds_layout<Sys, P, KAHAN>(CPL, a.SPB, a.UPS, a.sharedGlobals, a.sharedMatrix, a.cMatSize);
+    const int ldsPairs = mfma_lds_matrix<Sys, PAIRS>() ? a.ldsPairs : 0;
+    const LdsLayout L = lds_layout<Sys, P, KAHAN>(CPL, a.SPB, a.UPS, a.sharedGlobals, a.sharedMatrix, a.cMatSize, ldsPairs);
     S* sys = reinterpret_cast<S*>(mpgos_pb_lds);
     P* cptBuf = reinterpret_cast<P*>(mpgos_pb_lds + L.cpt);
     P* coupBuf = reinterpret_cast<P*>(mpgos_pb_lds + L.coup);
@@ -402,6 +527,21 @@ __device__ void pb_body(const Args<Sys, P, typename SysScope<Sys, P, KAHAN>::CP>
     }
     if (CPL == kMfmaCoupling)   // B operand: rows of absent systems stay zero
         for (int i = tid; i < NCp * kMfmaSystems * mfma_row(UPS); i += blockDim.x) cptBuf[i] = P(0);
+    const double* Ml = nullptr;
+    if (CPL == kMfmaCoupling && ldsPairs > 0) {
+        // the leading ldsPairs column-block pairs of every tile of the tiled
+        // matrix, for the whole Solve (16 bytes per lane and pair)
+        using Q = double2;
+        Q* lm = reinterpret_cast<Q*>(mpgos_pb_lds + L.mt);
+        const Q* src = reinterpret_cast<const Q*>(a.cTile);
+        const int lp = ldsPairs, pairs = UPS >> 3;
+        const int n = NCp * (UPS >> 4) * lp * 64;
+        for (int k = tid; k < n; k += blockDim.x) {
+            const int r = k >> 6;
+            lm[k] = src[((size_t)(r / lp) * pairs + r % lp) * 64 + (k & 63)];
+        }
+        Ml = reinterpret_cast<const double*>(lm);
+    }
     int cIndex[NCp];
 #pragma unroll
     for (int i = 0; i < NC; i++) cIndex[i] = a.cIndex[i];
@@ -511,7 +651,8 @@ __device__ void pb_body(const Args<Sys, P, typename SysScope<Sys, P, KAHAN>::CP>
             }
             __syncthreads();
             if (a.sharedMatrix) mfma_contract44<NC, false>(M, cptBuf, coupBuf, UPS, SPB);
-            else mfma_contract44<NC, true>(M, cptBuf, coupBuf, UPS, SPB, (const double*)a.cTile);
+            else mfma_contract44<NC, true, 8, mfma_tiles_together<Sys, PAIRS>()>(M, cptBuf, coupBuf, UPS, SPB, (const double*)a.cTile, Ml,
+                                                                                 ldsPairs);
             __syncthreads();
             if (run) {
 #pragma unroll

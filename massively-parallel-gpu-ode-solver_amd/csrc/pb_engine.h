@@ -13,6 +13,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -442,9 +443,22 @@ public:
         return mfma_applies() ? kMfmaCoupling : kLaneCoupling;
     }
 
-    size_t lds_bytes(int cpl, int spb, int sm) const
+    size_t lds_bytes(int cpl, int spb, int sm, int ldsPairs = 0) const
     {
-        return lds_layout<Sys, P, KAHAN>(cpl, spb, UPS, sharedGlobals, sm, cMatSize).total;
+        return lds_layout<Sys, P, KAHAN>(cpl, spb, UPS, sharedGlobals, sm, cMatSize, ldsPairs).total;
+    }
+
+    // Dense-contraction form with the tiled matrix: the leading column-block
+    // pairs of every tile that fit into the LDS the workgroup leaves free (a
+    // multiple of 4 pairs, one group of the contraction loop; at most all).
+    int lds_pairs(int spb) const
+    {
+        const size_t perPair = (size_t)std::max(NC, 1) * (UPS / 16) * 64 * 16;
+        const size_t base = lds_bytes(kMfmaCoupling, spb, 0, 1) - perPair;
+        if (base >= ldsMax) return 0;
+        int lp = (int)std::min<size_t>((ldsMax - base) / perPair, (size_t)(UPS / 8));
+        lp -= lp % 4;
+        return lp;
     }
 
     // PB-IF:1949-1965
@@ -461,7 +475,9 @@ public:
         a.SPB = spb;
         // the coupling matrix goes to LDS only when asked for and when it fits
         a.sharedMatrix = sharedMatrix && lds_bytes(cpl, spb, 1) <= ldsMax ? 1 : 0;
-        const size_t lds = lds_bytes(cpl, spb, a.sharedMatrix);
+        const bool tiled = !KAHAN && cpl == kMfmaCoupling && !a.sharedMatrix && std::is_same<P, double>::value;
+        if (tiled && (mfmaPairs == 2 ? mfma_lds_matrix<Sys, 2>() : mfma_lds_matrix<Sys, 4>())) a.ldsPairs = lds_pairs(spb);
+        const size_t lds = lds_bytes(cpl, spb, a.sharedMatrix, a.ldsPairs);
         if (lds > ldsMax) { set_error("per-block: system scope and coupling terms exceed the LDS of a workgroup"); return MPGOS_E_ARG; }
         const int threads = cpl == kMfmaCoupling ? kMfmaSystems * UPS / mfmaPairs : ((SPB * UPS + 63) / 64) * 64;
         const int grid = NS / spb + (NS % spb == 0 ? 0 : 1);
@@ -476,7 +492,7 @@ public:
                 else hipLaunchKernelGGL((pb_kernel<Sys, P, MPGOS_RKCK45, kMaxBlock, true>), dim3(grid), dim3(threads), lds, strm, a);
             }
         } else {
-            if (cpl == kMfmaCoupling && !a.sharedMatrix && std::is_same<P, double>::value) {
+            if (tiled) {
                 // the matrices in column-block pairs per lane, refreshed every Solve
                 // (a kernel on the solver's stream, so graph capture records it)
                 if (int rc = tile_pairs((const double*)cMat.d, cTile, cTileN)) return rc;
@@ -559,7 +575,8 @@ public:
             const int spb = cpl == kMfmaCoupling ? kMfmaSystems : SPB;
             a.SPB = spb;
             a.sharedMatrix = sharedMatrix && lds_bytes(cpl, spb, 1) <= ldsMax ? 1 : 0;
-            const size_t lds = lds_bytes(cpl, spb, a.sharedMatrix);
+            if (cpl == kMfmaCoupling && !a.sharedMatrix && mfma_lds_matrix<Sys, 4>()) a.ldsPairs = lds_pairs(spb);
+            const size_t lds = lds_bytes(cpl, spb, a.sharedMatrix, a.ldsPairs);
             if (lds > ldsMax) { set_error("per-block: system scope and coupling terms exceed the LDS of a workgroup"); return MPGOS_E_ARG; }
             const int threads = cpl == kMfmaCoupling ? kMfmaSystems * UPS / 4 : ((SPB * UPS + 63) / 64) * 64;
             const int grid = NS / spb + (NS % spb == 0 ? 0 : 1);
