@@ -510,6 +510,11 @@ __host__ __device__ inline RevMfmaLayout reverse_mfma_layout(int UPS)
 template <class Sys, int PAIRS>
 struct MfmaRevCtx {
     static constexpr int SPB = kMfmaSystems;
+    // tiles a wave contracts together in the replay and sweep kernels: all of
+    // them for every system (these kernels hold fewer values across the
+    // contraction than the forward step; ring with a quadrature state,
+    // reverse 352 -> 321 ms against one tile at a time)
+    static constexpr int TPW = PAIRS >= 4 ? PAIRS / 2 : 1;
     using Scope = RevScope<Sys>;
     Scope* sys; double* cptB; double* coup; int* blk;
     int lanesPerSys, lsys, u0, gsys, k, len, maxlen;
@@ -624,7 +629,7 @@ __device__ void pb_replay_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sy
             }
         }
         lds_barrier();
-        mfma_contract44<NC, true, 8, mfma_tiles_together<Sys, PAIRS>()>(a.cMat, cptB, coup, UPS, SPB, a.cTile);
+        mfma_contract44<NC, true, 8, MfmaRevCtx<Sys, PAIRS>::TPW>(a.cMat, cptB, coup, UPS, SPB, a.cTile);
         lds_barrier();
         if (run) {
 #pragma unroll
@@ -825,7 +830,7 @@ __device__ void pb_sweep_mfma_body(const Args<Sys, double>& a, const AdjArgs<Sys
                 }
             }
             __syncthreads();
-            mfma_contract44<NC, true, 8, mfma_tiles_together<Sys, PAIRS>()>(A.cMatT, cptB, coup, UPS, SPB, A.cTileT);
+            mfma_contract44<NC, true, 8, MfmaRevCtx<Sys, PAIRS>::TPW>(A.cMatT, cptB, coup, UPS, SPB, A.cTileT);
             __syncthreads();
 #pragma unroll
             for (int p = 0; p < PAIRS; p++) {
