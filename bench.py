@@ -253,8 +253,8 @@ def per_thread(args, mpgos, W, pyoracle):
     if sl and args.form == "relaunch":      # device-chained relaunches, budget --slice-steps per launch
         rcfg = {"mode": 2, "block": 64, "relaunch": sl, "launches": 64}
         forms["relaunch"] = time_form(mpgos, spec, rcfg, args.steps, 0, prec, 0, watch=True)
-    if sl and args.form == "sorted":        # pilot launch, device sort by estimated remaining steps, one final launch
-        scfg = {"mode": 3, "block": 64, "relaunch": 2000}
+    if sl and args.form == "sorted":        # pilot launch, device sort by estimated remaining steps, one final launch,
+        scfg = {"mode": 3, "block": 64, "relaunch": 2000, "graph": 1}   # all captured in one hipGraph
         forms["sorted"] = time_form(mpgos, spec, scfg, args.steps, 0, prec, 0, watch=True)
     if args.form in ("auto", "plain"):
         forms["plain"] = time_form(mpgos, spec, {"mode": 0, "block": 64}, args.steps, args.warmup, prec, sl)

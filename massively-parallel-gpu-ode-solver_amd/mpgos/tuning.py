@@ -97,6 +97,7 @@ def apply(solver, cfg):
     solver.SolverOption(O.PersistentSoloInstances, cfg.get("solo", 0))
     solver.SolverOption(O.RelaunchSteps, cfg.get("relaunch", 300))
     solver.SolverOption(O.RelaunchLaunches, cfg.get("launches", 16))
+    solver.SolverOption(O.GraphCapture, cfg.get("graph", 0))
 
 
 def time_config(solver, snap, cfg, repeats=2, warmup=1):

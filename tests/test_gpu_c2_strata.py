@@ -8,8 +8,9 @@ finishes in seconds and the longest instance needs about 3e6 attempted steps.
 Many instances then cross x' = 0 more often than the event log holds: the
 log keeps the first 256 hits while the count goes on (the detection path of
 SingleSystem_PerThread_Solver.cuh:323-345 with the hit record of the event
-log).  Every kernel form (plain, persistent, relaunch, sorted, host-sliced)
-must give the oracle's bits: states, times, accepted
+log).  Every kernel form (plain, persistent, relaunch, sorted, host-sliced;
+persistent and sorted also replayed from a captured hipGraph) must give the
+oracle's bits: states, times, accepted
 and rejected step counts, status, event counts and the logged hits.
 """
 import numpy as np
@@ -45,7 +46,7 @@ def strata(oracle):
     return spec, ref
 
 
-@pytest.mark.parametrize("form", ["plain", "persistent", "relaunch", "sorted", "sliced"])
+@pytest.mark.parametrize("form", ["plain", "persistent", "relaunch", "sorted", "sliced", "persistent-graph", "sorted-graph"])
 def test_c2_strata_bitwise(gpu, strata, form):
     spec, ref = strata
     print(f"C2 strata: {spec['nt']} instances, max {int(ref['nacc'].max())} accepted steps, "
@@ -58,6 +59,10 @@ def test_c2_strata_bitwise(gpu, strata, form):
         out = run_gpu(spec, kernel_mode=2, relaunch_steps=20000, relaunch_launches=40)
     elif form == "sorted":
         out = run_gpu(spec, kernel_mode=3, relaunch_steps=2000)
+    elif form == "persistent-graph":
+        out = run_gpu(spec, kernel_mode=1, graph=1)
+    elif form == "sorted-graph":        # pilot, device sort and final launch replayed from one hipGraph
+        out = run_gpu(spec, kernel_mode=3, relaunch_steps=2000, graph=1)
     else:
         out = run_gpu(spec, slice_steps=400000)
         assert out["_slices"] > 3
