@@ -2,7 +2,8 @@
 the reference tutorial's double-well ring (events, accessories, dense output,
 several systems per block), linear rings in every coupling storage form with
 the matrix in L2 or staged in LDS, partial activity, and the dense forced
-Duffing ring of 256 units.  Calls go through libmpgos's per-block C ABI."""
+Duffing ring of 256 units and of 96, 160 and 224 units (the dense-contraction
+form with the tiled matrix wholly or partly in LDS).  Calls go through libmpgos's per-block C ABI."""
 import numpy as np
 import pytest
 
@@ -151,6 +152,32 @@ def test_dense_duffing_ring_256(oracle, alg, kernel):
     for shm in (0, 1):
         got = run_pb_gpu(spec, shared_matrix=shm, kernel=kernel)
         assert_pb_bitwise(got, ref, f"shared_matrix={shm}")
+
+
+@pytest.mark.parametrize("ups", [96, 160, 224])
+def test_dense_ring_tile_counts(oracle, ups):
+    """Unit counts that are not powers of two: 6, 10 and 14 tiles, 3, 5 and 7
+    waves.  At 96 units the whole tiled matrix fits into the workgroup's LDS;
+    at 160 and 224 the leading 12 and 8 column-block pairs of every tile do and
+    the rest is read from global memory two groups deep.  Both units-per-lane
+    layouts of the dense-contraction form against the oracle and the lane
+    form, bit for bit."""
+    import pyoracle
+    spec = W.config4_ring(ns=11, ups=ups, periods=1)
+    spec["n_active"] = 10
+    print(f"dense ring ups={ups}", flush=True)
+    ref = pyoracle.run_pb("ring_duffing", spec)
+    assert_pb_bitwise(run_pb_gpu(spec, kernel=1), ref, f"lane ups={ups}")
+    assert_pb_bitwise(run_pb_gpu(spec, kernel=2), ref, f"mfma ups={ups}")
+    s = PBProblemSolver("ring_duffing", spec["ns"], ups, spec["algorithm"])
+    try:
+        W.load_pb(s, spec)
+        s.SolverOption(PBOption.CouplingKernel, 2)
+        s.SolverOption(PBOption.MfmaUnitsPerLane, 2)
+        s.solve_blocking()
+        assert_pb_bitwise(W.read_pb(s), ref, f"mfma 2 units per lane ups={ups}")
+    finally:
+        s.close()
 
 
 def test_second_solve_continues(oracle):
