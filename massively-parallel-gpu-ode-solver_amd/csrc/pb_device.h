@@ -271,7 +271,12 @@ struct Unit {
 // groups of G column blocks.  GLOBAL_M: M is in global memory, read with
 // global loads (a generic pointer gives flat loads, which also count as LDS
 // operations, so the wait for a group's B operands from LDS would wait for all
-// of its M loads too); otherwise M is in LDS.
+// of its M loads too); otherwise M is in LDS.  Mt: the pair-tiled copy of M
+// (tile_pair_matrix), read instead of M; Ml, ldsPairs: the leading ldsPairs
+// column-block pairs of every tile of Mt, copied into LDS.  TPW: the tiles a
+// wave contracts together when the launch gives every wave exactly TPW tiles
+// (2 * UPS threads at TPW = 2); then coup may alias cptB (results stored
+// after a workgroup barrier).  Otherwise coup and cptB must be distinct.
 template <int NC, bool GLOBAL_M = true, int G = 8, int TPW = 1>
 __device__ __forceinline__ void mfma_contract44(const double* M, const double* cptB, double* coup, int UPS, int SPB,
                                                 const double* Mt = nullptr, const double* Ml = nullptr, int ldsPairs = 0)
