@@ -78,12 +78,15 @@ template <class Sys, int PAIRS>
 __host__ __device__ constexpr bool mfma_lds_matrix() { return mfma_tiles_together<Sys, PAIRS>() > 1; }
 
 // LDS of the dense-contraction form: the B operand (coupling terms) and the
-// contraction results, one row of UPS + 4 doubles per (coupling, block
+// contraction results, one row of UPS + 2 doubles per (coupling, block
 // system).  A wave's writes and reads of one system's consecutive units fall
-// in consecutive banks, and the B reads of a column block (4 columns x 4
-// systems, rows UPS + 4 apart: 8 banks apart per system) in 16 distinct bank
-// pairs.
-__host__ __device__ constexpr int mfma_row(int UPS) { return UPS + 4; }
+// in consecutive banks; rows 4 banks apart (UPS a multiple of 16) put the B
+// reads of a column block (4 columns x 4 systems, two columns per half-wave)
+// in 8 distinct bank pairs per half-wave and spread the result stores (4
+// systems x 16 rows of a tile) evenly over the 32 banks (UPS + 4, 8 banks
+// apart, stored them 4 ways conflicting: twice the measured bank conflicts,
+// 1.4 % slower).
+__host__ __device__ constexpr int mfma_row(int UPS) { return UPS + 2; }
 __host__ __device__ inline size_t mfma_index(int i, int s, int u, int UPS)
 {
     return ((size_t)i * kMfmaSystems + s) * mfma_row(UPS) + u;
@@ -796,9 +799,10 @@ __device__ void pb_body(const Args<Sys, P, typename SysScope<Sys, P, KAHAN>::CP>
         if (lane && my->term == 0 && !isfin) my->isfin = 0;
         // ---- controller (PB-CTL:424-542)
         if (ALG == MPGOS_RKCK45) {
-            if (lane && my->term == 0) {
-                CP R = CP(1e30);
-                int upd = 1;
+            const bool part = lane && my->term == 0;
+            CP R = CP(1e30);                    // the reduction's start value, also for lanes not taking part
+            int upd = 1;
+            if (part) {
 #pragma unroll
                 for (int p = 0; p < PAIRS; p++)
 #pragma unroll
@@ -807,6 +811,25 @@ __device__ void pb_body(const Args<Sys, P, typename SysScope<Sys, P, KAHAN>::CP>
                         upd = upd & (err[p][c] < tol);
                         R = dmin<CP>(R, tol / err[p][c]);
                     }
+            }
+            // systems on aligned power-of-two groups of lanes (or whole waves):
+            // reduce in the group with lane shuffles, then one LDS atomic per
+            // group instead of one per lane (same-address atomics serialise).
+            // min and AND do not depend on the order, so the bits are those of
+            // the per-lane atomics.  Only where the lane's stage vectors leave
+            // registers for it (the two-tile dense-contraction kernels: ring of
+            // 256 two-component units 107.2 -> 106.5 ms; with three components
+            // the extra registers spilled, 125.8 -> 131.3 ms).
+            const int grp = !(CPL == kMfmaCoupling && mfma_tiles_together<Sys, PAIRS>() > 1) ? 0
+                            : lanesPerSys >= 64 ? (lanesPerSys % 64 == 0 ? 64 : 0)
+                                                : ((lanesPerSys & (lanesPerSys - 1)) == 0 ? lanesPerSys : 0);
+            if (grp > 1) {
+                for (int m = 1; m < grp; m <<= 1) {
+                    R = dmin<CP>(R, __shfl_xor(R, m));
+                    upd &= __shfl_xor(upd, m);
+                }
+            }
+            if (part && (grp == 0 || (tid & (grp - 1)) == 0)) {
                 atomicAnd(&my->update, upd);
                 atomicMin(&my->ratioBits, order_bits<CP>(R));
             }
