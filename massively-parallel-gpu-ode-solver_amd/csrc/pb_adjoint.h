@@ -489,7 +489,7 @@ __host__ __device__ inline RevMfmaLayout reverse_mfma_layout(int UPS)
 
 // Reverse pass in the dense-contraction form (full FP64 coupling matrix, UPS a
 // multiple of 32 up to 256): 8 systems per workgroup, 4 units per lane
-// (2 * UPS threads), every coupling product on v_mfma_f64_16x16x4f64 --
+// (2 * UPS threads), every coupling product on v_mfma_f64_4x4x4f64 (mfma_contract44) --
 // M CPT(Y) in the segment replay, M^T w (cMatT) in the adjoint sweep.  Round r
 // handles the r-th segment from the end of every system in two launches:
 //   pb_replay_mfma_kernel: restore checkpoint k = nseg - 1 - r and replay the
