@@ -97,7 +97,8 @@ def apply(solver, cfg):
     solver.SolverOption(O.PersistentSoloInstances, cfg.get("solo", 0))
     solver.SolverOption(O.RelaunchSteps, cfg.get("relaunch", 300))
     solver.SolverOption(O.RelaunchLaunches, cfg.get("launches", 16))
-    solver.SolverOption(O.GraphCapture, cfg.get("graph", 0))
+    if "graph" in cfg:                          # else the solver's own GraphCapture setting stays
+        solver.SolverOption(O.GraphCapture, cfg["graph"])
 
 
 def time_config(solver, snap, cfg, repeats=2, warmup=1):
