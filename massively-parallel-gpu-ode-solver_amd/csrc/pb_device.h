@@ -70,8 +70,7 @@ constexpr int kMfmaPairs = 4;            // default pairs per lane of the dense-
 // Tiles of the dense-contraction form that a wave contracts together: all of
 // its PAIRS / 2 tiles when the lane's unit states and stage vectors leave room
 // in the registers (UD * PAIRS <= 8), else one at a time.  Only the first way
-// keeps leading matrix pairs in LDS (the results then reuse the rows of the
-// coupling terms, which frees the LDS for them).
+// keeps leading matrix pairs in LDS.
 template <class Sys, int PAIRS>
 __host__ __device__ constexpr int mfma_tiles_together() { return PAIRS >= 4 && Sys::UD * PAIRS <= 8 ? PAIRS / 2 : 1; }
 template <class Sys, int PAIRS>
@@ -179,10 +178,7 @@ __host__ __device__ inline LdsLayout lds_layout(int cpl, int SPB, int UPS, int s
     if (cpl == kMfmaCoupling) {
         o += (size_t)NCp * kMfmaSystems * mfma_row(UPS) * sizeof(P);
         L.coup = o;
-        // with matrix pairs in LDS the contraction results overwrite the
-        // coupling terms once every wave has finished reading them
-        if (ldsPairs > 0) L.coup = L.cpt;
-        else o += (size_t)NCp * kMfmaSystems * mfma_row(UPS) * sizeof(P);
+        o += (size_t)NCp * kMfmaSystems * mfma_row(UPS) * sizeof(P);
     } else {
         o += 2 * (size_t)SPB * UPS * NCp * sizeof(P);
         L.coup = o;
@@ -278,8 +274,7 @@ struct Unit {
 // (tile_pair_matrix), read instead of M; Ml, ldsPairs: the leading ldsPairs
 // column-block pairs of every tile of Mt, copied into LDS.  TPW: the tiles a
 // wave contracts together when the launch gives every wave exactly TPW tiles
-// (2 * UPS threads at TPW = 2); then coup may alias cptB (results stored
-// after a workgroup barrier).  Otherwise coup and cptB must be distinct.
+// (2 * UPS threads at TPW = 2).
 template <int NC, bool GLOBAL_M = true, int G = 8, int TPW = 1>
 __device__ __forceinline__ void mfma_contract44(const double* M, const double* cptB, double* coup, int UPS, int SPB,
                                                 const double* Mt = nullptr, const double* Ml = nullptr, int ldsPairs = 0)
@@ -310,11 +305,10 @@ __device__ __forceinline__ void mfma_contract44(const double* M, const double* c
         // blocks per tile and group: one B read per column block and chain
         // serves all of them, and the 2 TPW accumulator chains are
         // independent.  The leading ldsPairs pairs of every tile (a multiple
-        // of G / 2) come from the workgroup's LDS copy (Ml, same layout),
-        // which takes their loads off the vector-memory path; the global
-        // groups are read two buffers deep, the first during the LDS part.
-        // coup == cptB: the results overwrite the B operand, so they are
-        // stored after every wave of the workgroup has finished.
+        // of G / (2 TPW)) come from the workgroup's LDS copy (Ml, same
+        // layout), which takes their loads off the vector-memory path; the
+        // global groups are read two buffers deep, the first during the LDS
+        // part.
         constexpr int TT = TPW > 1 ? TPW : 2;
         constexpr int GB = G / TT;                      // column blocks per tile and group
         static_assert(GB % 2 == 0, "whole pairs per group");
@@ -387,7 +381,6 @@ __device__ __forceinline__ void mfma_contract44(const double* M, const double* c
                 }
             }
         }
-        if (coup == cptB) lds_barrier();
 #pragma unroll
         for (int i = 0; i < NC; i++)
 #pragma unroll

@@ -450,14 +450,17 @@ public:
 
     // Dense-contraction form with the tiled matrix: the leading column-block
     // pairs of every tile that fit into the LDS the workgroup leaves free (a
-    // multiple of 4 pairs, one group of the contraction loop; at most all).
+    // multiple of 2 pairs, one group of the two-tile contraction loop; at
+    // most all).  At U = 256: 6 of 32 pairs, 96 KiB.  (Letting the results
+    // overwrite the coupling terms after a barrier made room for 8 pairs but
+    // ran 1.4 % slower.)
     int lds_pairs(int spb) const
     {
         const size_t perPair = (size_t)std::max(NC, 1) * (UPS / 16) * 64 * 16;
         const size_t base = lds_bytes(kMfmaCoupling, spb, 0, 1) - perPair;
         if (base >= ldsMax) return 0;
         int lp = (int)std::min<size_t>((ldsMax - base) / perPair, (size_t)(UPS / 8));
-        lp -= lp % 4;
+        lp -= lp % 2;
         return lp;
     }
 
